@@ -1,0 +1,202 @@
+// extern "C" boundary of the library (declarations and reference citations in include/hdd.h).
+#include "../../include/hdd.h"
+#include "hdd_runtime.h"
+#include "hdd_sigma_device.h"
+#include <cstring>
+#include <string>
+
+using namespace hdd;
+
+struct hdd_ctx {
+    std::unique_ptr<Solver> solver;
+    std::string err;
+};
+
+static thread_local std::string g_create_error;
+
+#define HDD_TRY(ctx, body)                                  \
+    try {                                                   \
+        body;                                               \
+        return HDD_OK;                                      \
+    } catch (std::exception & e) {                          \
+        if (ctx) (ctx)->err = e.what();                     \
+        return HDD_ERR_DEVICE;                              \
+    }
+
+extern "C" {
+
+hdd_status hdd_create(const hdd_problem_desc* d, hdd_ctx** out) {
+    if (!d || !out || d->n_pages < 0 || (d->n_pages > 0 && (!d->page_offsets || !d->flags || !d->b))) {
+        g_create_error = "hdd_create: invalid arguments";
+        return HDD_ERR_ARG;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (d->size[a] < 2 || d->sub_size[a] < 2) { g_create_error = "hdd_create: size and sub_size must be >= 2"; return HDD_ERR_ARG; }
+    if (d->vlevels < 1 || d->sigma_levels < 1 || d->nVcycles < 0 || d->interior_smoothing < 0 || d->boundary_smoothing < 0 ||
+        d->sigma_smoothing < 0 || d->sigma_mg < 0) {
+        g_create_error = "hdd_create: invalid solver parameters";
+        return HDD_ERR_ARG;
+    }
+    try {
+        SolverParams p;
+        p.size = {d->size[0], d->size[1], d->size[2]};
+        p.sub = {d->sub_size[0], d->sub_size[1], d->sub_size[2]};
+        p.vlevels = d->vlevels; p.sigma_levels = d->sigma_levels; p.nV = d->nVcycles;
+        p.int_s = d->interior_smoothing; p.bnd_s = d->boundary_smoothing;
+        p.sig_s = d->sigma_smoothing; p.sig_mg = d->sigma_mg;
+        auto* c = new hdd_ctx();
+        try {
+            c->solver.reset(new Solver(p, d->n_pages, d->page_offsets, d->flags, d->b));
+        } catch (...) { delete c; throw; }
+        *out = c;
+        return HDD_OK;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return HDD_ERR_SETUP;
+    }
+}
+
+hdd_status hdd_destroy(hdd_ctx* ctx) {
+    if (!ctx) return HDD_ERR_ARG;
+    delete ctx;
+    return HDD_OK;
+}
+
+const char* hdd_last_error(const hdd_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+hdd_status hdd_num_pages(const hdd_ctx* ctx, int* n) {
+    if (!ctx || !n) return HDD_ERR_ARG;
+    *n = ctx->solver->lv[0].host.npages();
+    return HDD_OK;
+}
+
+hdd_status hdd_page_offsets(const hdd_ctx* ctx, uint64_t* out) {
+    if (!ctx || !out) return HDD_ERR_ARG;
+    const PagedLevel& L = ctx->solver->lv[0].host;
+    for (int p = 0; p < L.npages(); ++p) out[p] = page_key(L.page_base[p].x >> 2, L.page_base[p].y >> 3, L.page_base[p].z >> 3) << 12;
+    return HDD_OK;
+}
+
+hdd_status hdd_set_channel(hdd_ctx* ctx, int ch, const float* host) {
+    if (!ctx || !host || ch < 0 || ch > 6) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        Solver& S = *ctx->solver;
+        const size_t n = (size_t)S.lv[0].host.npages() * kPageCells;
+        hdd_check(hipMemcpy(S.channel(ch), host, n * sizeof(float), hipMemcpyHostToDevice), "set_channel");
+    })
+}
+
+hdd_status hdd_get_channel(hdd_ctx* ctx, int ch, float* host) {
+    if (!ctx || !host || ch < 0 || ch > 6) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        Solver& S = *ctx->solver;
+        hdd_check(hipStreamSynchronize(S.stream), "sync");
+        const size_t n = (size_t)S.lv[0].host.npages() * kPageCells;
+        hdd_check(hipMemcpy(host, S.channel(ch), n * sizeof(float), hipMemcpyDeviceToHost), "get_channel");
+    })
+}
+
+hdd_status hdd_level_pages(const hdd_ctx* ctx, int level, int* n) {
+    if (!ctx || !n || level < 0 || level >= (int)ctx->solver->lv.size()) return HDD_ERR_ARG;
+    *n = ctx->solver->lv[level].host.npages();
+    return HDD_OK;
+}
+
+hdd_status hdd_level_flags(const hdd_ctx* ctx, int level, uint64_t* offsets, uint32_t* flags) {
+    if (!ctx || level < 0 || level >= (int)ctx->solver->lv.size()) return HDD_ERR_ARG;
+    const PagedLevel& L = ctx->solver->lv[level].host;
+    for (int p = 0; p < L.npages(); ++p) {
+        if (offsets) offsets[p] = page_key(L.page_base[p].x >> 2, L.page_base[p].y >> 3, L.page_base[p].z >> 3) << 12;
+        if (flags) std::memcpy(flags + (size_t)p * kPageCells, &L.flags[(size_t)p * kPageCells], kPageCells * sizeof(uint32_t));
+    }
+    return HDD_OK;
+}
+
+hdd_status hdd_smooth(hdd_ctx* ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 2) return HDD_ERR_ARG;
+    HDD_TRY(ctx, { ctx->solver->smooth0(mode); hdd_check(hipStreamSynchronize(ctx->solver->stream), "smooth"); })
+}
+
+hdd_status hdd_vcycle(hdd_ctx* ctx) {
+    if (!ctx) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        Solver& S = *ctx->solver;
+        launch_fill(S.lv[0].u[S.lv[0].cur].p, 0.f, S.lv[0].cells(), S.stream);
+        S.vcycle();
+        hdd_check(hipStreamSynchronize(S.stream), "vcycle");
+    })
+}
+
+hdd_status hdd_apply_preconditioner(hdd_ctx* ctx) {
+    if (!ctx) return HDD_ERR_ARG;
+    HDD_TRY(ctx, { ctx->solver->apply_preconditioner(); hdd_check(hipStreamSynchronize(ctx->solver->stream), "apply"); })
+}
+
+hdd_status hdd_sigma_info(const hdd_ctx* ctx, int* levels, int* nsub) {
+    if (!ctx) return HDD_ERR_ARG;
+    if (levels) *levels = ctx->solver->sigma_host.levels;
+    if (nsub) *nsub = ctx->solver->sigma_host.nsub;
+    return HDD_OK;
+}
+
+hdd_status hdd_interface_dofs(const hdd_ctx* ctx, int level, int* n) {
+    if (!ctx || !n || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
+    *n = ctx->solver->sigma_host.n_if[level];
+    return HDD_OK;
+}
+
+hdd_status hdd_interface_coords(const hdd_ctx* ctx, int level, int* xyz) {
+    if (!ctx || !xyz || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
+    const auto& v = ctx->solver->sigma_host.if_coords[level];
+    for (size_t i = 0; i < v.size(); ++i) { xyz[3 * i] = v[i].x; xyz[3 * i + 1] = v[i].y; xyz[3 * i + 2] = v[i].z; }
+    return HDD_OK;
+}
+
+hdd_status hdd_interface_matrix(const hdd_ctx* ctx, int which, int level, int sd, int* rows, int* cols, int* nnz, int* ptr, int* idx, float* val) {
+    if (!ctx) return HDD_ERR_ARG;
+    const SigmaHost& S = ctx->solver->sigma_host;
+    if (level < 0 || level >= S.levels || which < 0 || which > 3) return HDD_ERR_ARG;
+    if ((which == 1 || which == 2) && (sd < 0 || sd >= S.nsub)) return HDD_ERR_ARG;
+    if (which == 3 && level >= S.levels - 1) return HDD_ERR_ARG;
+    const Csr& M = which == 0 ? S.Arr[level] : which == 1 ? S.Aii[level][sd] : which == 2 ? S.Air[level][sd] : S.Prr[level];
+    if (rows) *rows = M.rows;
+    if (cols) *cols = M.cols;
+    if (nnz) *nnz = (int)M.nnz();
+    if (ptr) std::memcpy(ptr, M.ptr.data(), M.ptr.size() * sizeof(int));
+    if (idx) std::memcpy(idx, M.idx.data(), M.idx.size() * sizeof(int));
+    if (val) std::memcpy(val, M.val.data(), M.val.size() * sizeof(float));
+    return HDD_OK;
+}
+
+hdd_status hdd_sigma_vcycle(hdd_ctx* ctx, const float* b, float* u) {
+    if (!ctx || !b || !u) return HDD_ERR_ARG;
+    HDD_TRY(ctx, { sigma_vcycle_host(*ctx->solver->sigma, b, u, ctx->solver->stream); })
+}
+
+hdd_status hdd_sigma_smooth(hdd_ctx* ctx, int level, const float* b, float* u) {
+    if (!ctx || !b || !u || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
+    HDD_TRY(ctx, { sigma_smooth_host(*ctx->solver->sigma, level, b, u, ctx->solver->stream); })
+}
+
+hdd_status hdd_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_report* rep, double* hist, int hist_len) {
+    if (!ctx || !prm || prm->max_iterations < 0) return HDD_ERR_ARG;
+    try {
+        std::vector<double> h;
+        double secs = 0;
+        int it = ctx->solver->pcg(prm->tolerance, prm->max_iterations, h, &secs);
+        if (rep) { rep->iterations = it; rep->seconds = secs; rep->final_norm = h.empty() ? 0.0 : h.back(); }
+        if (hist) for (int i = 0; i < (int)h.size() && i < hist_len; ++i) hist[i] = h[i];
+        return it < 0 ? HDD_ERR_NOT_CONVERGED : HDD_OK;
+    } catch (std::exception& e) {
+        ctx->err = e.what();
+        return HDD_ERR_DEVICE;
+    }
+}
+
+hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* s) {
+    if (!ctx || !s) return HDD_ERR_ARG;
+    *s = ctx->solver->setup_seconds;
+    return HDD_OK;
+}
+
+}  // extern "C"

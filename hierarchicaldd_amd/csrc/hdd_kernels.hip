@@ -1,0 +1,299 @@
+// HIP kernels for gfx950: matrix-free V-cycle on paged SPGrid levels and the
+// outer PCG vector operations.
+//
+// Launch shape: one 256-thread workgroup (four 64-lane wavefronts) per 4x8x8
+// page; thread t owns cell t of its page (t = x*64 + y*8 + z), so every
+// channel access of a wavefront is a contiguous, coalesced 256-byte segment.
+// Face neighbours inside the page are at t +-64, +-8, +-1; across the page
+// face they come from the page's 6-entry neighbour table (the zero page
+// `np` stands in for unallocated pages).
+//
+// Floating-point order reproduces the reference kernels operation by
+// operation (the library is compiled with -ffp-contract=off):
+//   residual   r = b - sum_f -(u_n - u_c)   LIB/SPGrid_Residual_Legacy.h:52-61
+//   correction u += (omega/fc) * r          LIB/SPGrid_Correction_Legacy.h:60-77
+//   restriction / prolongation weights      LIB/SPGrid_Restriction_Legacy.h:50-181,
+//                                           LIB/SPGrid_Prolongation_Legacy.h:48-123
+//   PCG Laplace (double accumulation)       DDS/SPGrid_Gemini_Arithmetic_Helpers.h:240-262
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "hdd_device.h"
+
+namespace hdd {
+
+__constant__ float c_omega_fc[8];   // (float)(2/3) / (float)fc, fc = 0..6
+__constant__ float c_rweights[64];  // restriction weights, [i][j][k] with w = {1,3,3,1}/8
+__constant__ float c_pweights[8];   // prolongation weights 4 * {3/4,1/4}^3, [i][j][k]
+
+__device__ __forceinline__ int nb_index(const int* __restrict__ nbr, int p, int c, int f) {
+    switch (f) {
+        case 0: return (c >= 64) ? p * 256 + c - 64 : nbr[p * 6 + 0] * 256 + c + 192;
+        case 1: return (c < 192) ? p * 256 + c + 64 : nbr[p * 6 + 1] * 256 + c - 192;
+        case 2: return ((c & 56) != 0) ? p * 256 + c - 8 : nbr[p * 6 + 2] * 256 + c + 56;
+        case 3: return ((c & 56) != 56) ? p * 256 + c + 8 : nbr[p * 6 + 3] * 256 + c - 56;
+        case 4: return ((c & 7) != 0) ? p * 256 + c - 1 : nbr[p * 6 + 4] * 256 + c + 7;
+        default: return ((c & 7) != 7) ? p * 256 + c + 1 : nbr[p * 6 + 5] * 256 + c - 7;
+    }
+}
+
+// float residual of one cell, reference operation order
+__device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int p, int c, uint32_t f,
+                                               const float* __restrict__ u, float b) {
+    const float cv = u[p * 256 + c];
+    float result = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if (f & (kFaceMX << k)) {
+            const float nv = u[nb_index(nbr, p, c, k)];
+            result -= (nv - cv);
+        }
+    return b - result;
+}
+
+// One damped-Jacobi sweep (residual + correction fused, ping-pong buffers).
+// Cells outside the selection are copied unchanged.
+__global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restrict__ u_in, float* __restrict__ u_out,
+                                                const float* __restrict__ b, int write_interior, int write_boundary) {
+    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const uint32_t f = L.flags[e];
+    const uint32_t sel = f & (kActive | kBoundary);
+    const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
+    float u = u_in[e];
+    if (w) {
+        const float r = cell_residual(L.nbr, p, c, f, u_in, b[e]);
+        u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * r;
+    }
+    u_out[e] = u;
+}
+
+__global__ void __launch_bounds__(256) k_residual(DevLevel L, const float* __restrict__ u, const float* __restrict__ b,
+                                                  float* __restrict__ r) {
+    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const uint32_t f = L.flags[e];
+    if (f & kActive) r[e] = cell_residual(L.nbr, p, c, f, u, b[e]);
+}
+
+__global__ void k_fill(float* __restrict__ a, float v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+}
+__global__ void k_copy(float* __restrict__ d, const float* __restrict__ s, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+
+__device__ __forceinline__ int lut_page(const DevLevel& L, int x, int y, int z) {
+    const int bx = x >> 2, by = y >> 3, bz = z >> 3;
+    if (x < 0 || y < 0 || z < 0 || bx >= L.nbx || by >= L.nby || bz >= L.nbz) return L.np;
+    const int q = L.lut[(bx * L.nby + by) * L.nbz + bz];
+    return q < 0 ? L.np : q;
+}
+
+// coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only)
+__global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const float* __restrict__ r_f, float* __restrict__ b_c) {
+    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    if (!(C.flags[e] & kActive)) return;
+    const int x = C.base[p * 3 + 0] + (c >> 6), y = C.base[p * 3 + 1] + ((c >> 3) & 7), z = C.base[p * 3 + 2] + (c & 7);
+    const int fx = 2 * x - 2, fy = 2 * y - 2, fz = 2 * z - 2;
+    float acc = 0.f;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            for (int k = 0; k < 4; ++k) {
+                const int X = fx + i, Y = fy + j, Z = fz + k;
+                const int q = lut_page(F, X, Y, Z);
+                acc += c_rweights[i * 16 + j * 4 + k] * r_f[q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7))];
+            }
+    b_c[e] = acc;
+}
+
+// fine u += sum_{2x2x2} w u_coarse (Active fine cells only)
+__global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, float* __restrict__ u_f, const float* __restrict__ u_c) {
+    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    if (!(F.flags[e] & kActive)) return;
+    const int x = F.base[p * 3 + 0] + (c >> 6), y = F.base[p * 3 + 1] + ((c >> 3) & 7), z = F.base[p * 3 + 2] + (c & 7);
+    const int cx = (x + 1) >> 1, cy = (y + 1) >> 1, cz = (z + 1) >> 1;
+    const int px = 1 - (x - (2 * cx - 1)), py = 1 - (y - (2 * cy - 1)), pz = 1 - (z - (2 * cz - 1));
+    float u = u_f[e];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k) {
+                const int X = cx + i - px, Y = cy + j - py, Z = cz + k - pz;
+                const int q = lut_page(C, X, Y, Z);
+                u += c_pweights[((i ^ px) << 2) | ((j ^ py) << 1) | (k ^ pz)] * u_c[q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7))];
+            }
+    u_f[e] = u;
+}
+
+// PCG operator: out = L v on Active|Interface cells (double accumulation), 0 elsewhere (fused Project)
+__global__ void __launch_bounds__(256) k_laplace_pcg(DevLevel L, const float* __restrict__ v, float* __restrict__ out) {
+    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const uint32_t f = L.flags[e];
+    float res = 0.f;
+    if (f & (kActive | kInterface)) {
+        const double cv = (double)v[e];
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (f & (kFaceMX << k)) acc -= ((double)v[nb_index(L.nbr, p, c, k)] - cv);
+        res = (float)acc;
+    }
+    out[e] = res;
+}
+
+// Project: zero values outside Active|Interface
+__global__ void __launch_bounds__(256) k_project(DevLevel L, float* __restrict__ v) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (!(L.flags[e] & (kActive | kInterface))) v[e] = 0.f;
+}
+
+// r = b - q, then Project
+__global__ void __launch_bounds__(256) k_sub_project(DevLevel L, const float* __restrict__ b, const float* __restrict__ q, float* __restrict__ r) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    r[e] = (L.flags[e] & (kActive | kInterface)) ? b[e] - q[e] : 0.f;
+}
+
+// out = in1 * c + in2 (SPGrid_Copy2), c read from device scalar slot
+__global__ void __launch_bounds__(256) k_copy2(float* __restrict__ out, const float* __restrict__ in1, const float* __restrict__ in2,
+                                               const float* __restrict__ cptr, float csign, size_t n) {
+    const float cc = csign * cptr[0];
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in1[i] * cc + in2[i];
+}
+
+// x = s*alpha + x ; r = q*(-alpha) + r  (two SPGrid_Copy2 calls fused)
+__global__ void __launch_bounds__(256) k_xr_update(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ s,
+                                                   const float* __restrict__ q, const float* __restrict__ alpha, size_t n) {
+    const float a = alpha[0], na = -a;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        x[i] = s[i] * a + x[i];
+        r[i] = q[i] * na + r[i];
+    }
+}
+
+// block-partial double dot products of float products (a*b rounded to float, then summed in double)
+__global__ void __launch_bounds__(256) k_dot_partial(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ part, size_t n) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) s += (double)(a[i] * b[i]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+__global__ void __launch_bounds__(256) k_maxabs_partial(const float* __restrict__ a, float* __restrict__ part, size_t n) {
+    __shared__ float sh[4];
+    float m = 0.f;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) m = fmaxf(m, fabsf(a[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+}
+
+// Final reduction of the partials and the PCG scalar logic of
+// CONJUGATE_GRADIENT_DUMPING::Solve (:193-205) in float:
+//   mode 0: rho = (float)dot -> scal[0]; beta = rho/rho_old -> scal[2]
+//   mode 1: s_dot_q = (float)dot; alpha = rho/s_dot_q -> scal[1]; rho_old = rho -> scal[3]
+//   mode 2: max-abs norm -> scal[4]
+__global__ void __launch_bounds__(256) k_finish(const double* __restrict__ dpart, const float* __restrict__ fpart, int nparts, int mode,
+                                                float* __restrict__ scal, double* __restrict__ dscal) {
+    __shared__ double sd[256];
+    __shared__ float sf[256];
+    double s = 0.0;
+    float m = 0.f;
+    for (int i = threadIdx.x; i < nparts; i += 256) {
+        if (mode < 2) s += dpart[i]; else m = fmaxf(m, fpart[i]);
+    }
+    sd[threadIdx.x] = s; sf[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sd[threadIdx.x] += sd[threadIdx.x + o]; sf[threadIdx.x] = fmaxf(sf[threadIdx.x], sf[threadIdx.x + o]); }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (mode == 0) {
+            const float rho = (float)sd[0];
+            scal[0] = rho;
+            scal[2] = rho / scal[3];
+            dscal[0] = sd[0];
+        } else if (mode == 1) {
+            const float sdq = (float)sd[0];
+            const float rho = scal[0];
+            scal[1] = sdq != 0.f ? rho / sdq : 3.40282347e+38f;
+            scal[3] = rho;
+            dscal[1] = sd[0];
+        } else {
+            scal[4] = sf[0];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+static inline int grid_for(size_t n) { size_t g = (n + 255) / 256; return (int)(g > 4096 ? 4096 : (g == 0 ? 1 : g)); }
+
+void upload_constants() {
+    float om[8];
+    const float omega = (float)(2. / 3.);
+    om[0] = 0.f;
+    for (int k = 1; k < 8; ++k) om[k] = omega / (float)k;
+    float s1[4] = {(float)(1. / 8.), (float)(3. / 8.), (float)(3. / 8.), (float)(1. / 8.)};
+    float rw[64];
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) for (int k = 0; k < 4; ++k) rw[i * 16 + j * 4 + k] = s1[i] * s1[j] * s1[k];
+    float s2[2] = {0.75f, 0.25f};
+    float pw[8];
+    for (int i = 0; i < 2; ++i) for (int j = 0; j < 2; ++j) for (int k = 0; k < 2; ++k) pw[i * 4 + j * 2 + k] = s2[i] * s2[j] * s2[k] * 4.0f;
+    hdd_check(hipMemcpyToSymbol(HIP_SYMBOL(c_omega_fc), om, sizeof(om)));
+    hdd_check(hipMemcpyToSymbol(HIP_SYMBOL(c_rweights), rw, sizeof(rw)));
+    hdd_check(hipMemcpyToSymbol(HIP_SYMBOL(c_pweights), pw, sizeof(pw)));
+}
+
+void launch_smooth(const DevLevel& L, const float* u_in, float* u_out, const float* b, bool interior, bool boundary, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_smooth, dim3(L.np), dim3(256), 0, s, L, u_in, u_out, b, (int)interior, (int)boundary);
+}
+void launch_residual(const DevLevel& L, const float* u, const float* b, float* r, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_residual, dim3(L.np), dim3(256), 0, s, L, u, b, r);
+}
+void launch_fill(float* a, float v, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, a, v, n);
+}
+void launch_copy(float* d, const float* src, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, d, src, n);
+}
+void launch_restrict(const DevLevel& F, const DevLevel& C, const float* r_f, float* b_c, hipStream_t s) {
+    if (C.np) hipLaunchKernelGGL(k_restrict, dim3(C.np), dim3(256), 0, s, F, C, r_f, b_c);
+}
+void launch_prolong(const DevLevel& F, const DevLevel& C, float* u_f, const float* u_c, hipStream_t s) {
+    if (F.np) hipLaunchKernelGGL(k_prolong, dim3(F.np), dim3(256), 0, s, F, C, u_f, u_c);
+}
+void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_laplace_pcg, dim3(L.np), dim3(256), 0, s, L, v, out);
+}
+void launch_project(const DevLevel& L, float* v, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_project, dim3(L.np), dim3(256), 0, s, L, v);
+}
+void launch_sub_project(const DevLevel& L, const float* b, const float* q, float* r, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_sub_project, dim3(L.np), dim3(256), 0, s, L, b, q, r);
+}
+void launch_copy2(float* out, const float* in1, const float* in2, const float* cptr, float csign, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy2, dim3(grid_for(n)), dim3(256), 0, s, out, in1, in2, cptr, csign, n);
+}
+void launch_xr_update(float* x, float* r, const float* sv, const float* q, const float* alpha, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_xr_update, dim3(grid_for(n)), dim3(256), 0, s, x, r, sv, q, alpha, n);
+}
+int reduce_parts(size_t n) { return grid_for(n) > 1024 ? 1024 : grid_for(n); }
+void launch_dot(const float* a, const float* b, size_t n, double* part, int mode, float* scal, double* dscal, hipStream_t s) {
+    const int g = reduce_parts(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(256), 0, s, a, b, part, n);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, (const float*)nullptr, g, mode, scal, dscal);
+}
+void launch_maxabs(const float* a, size_t n, float* part, float* scal, hipStream_t s) {
+    const int g = reduce_parts(n);
+    hipLaunchKernelGGL(k_maxabs_partial, dim3(g), dim3(256), 0, s, a, part, n);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, (const double*)nullptr, part, g, 2, scal, (double*)nullptr);
+}
+
+}  // namespace hdd

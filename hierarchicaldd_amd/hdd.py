@@ -1,0 +1,215 @@
+"""ctypes binding of the C ABI in include/hdd.h (library hierarchicaldd_amd/_build/libhdd.so).
+
+The binding raises if the library has not been built; there is no CPU
+fallback. Method names mirror the reference facade
+(SPGrid_Domain_Decomposition_Solver: Read_SPGrid / Initialize_Preconditioner /
+Solve, DDS/SPGrid_Domain_Decomposition_Solver.h:73-197).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libhdd.so")
+_lib = None
+
+CH_Z, CH_R, CH_K, CH_X, CH_B, CH_Q, CH_S = range(7)
+
+
+class HddError(RuntimeError):
+    pass
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [("size", C.c_int * 3), ("sub_size", C.c_int * 3), ("vlevels", C.c_int), ("sigma_levels", C.c_int),
+                ("nVcycles", C.c_int), ("interior_smoothing", C.c_int), ("boundary_smoothing", C.c_int),
+                ("sigma_smoothing", C.c_int), ("sigma_mg", C.c_int), ("n_pages", C.c_int),
+                ("page_offsets", C.c_void_p), ("flags", C.c_void_p), ("b", C.c_void_p)]
+
+
+class PcgParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int)]
+
+
+class PcgReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("final_norm", C.c_double)]
+
+
+EXPORTS = ["hdd_create", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
+           "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
+           "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
+           "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds"]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HddError(f"{LIB_PATH} is missing: build it with `python -m hierarchicaldd_amd.build` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        P, I = C.c_void_p, C.c_int
+        L.hdd_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
+        L.hdd_destroy.argtypes = [P]
+        L.hdd_last_error.argtypes = [P]
+        L.hdd_last_error.restype = C.c_char_p
+        L.hdd_num_pages.argtypes = [P, C.POINTER(I)]
+        L.hdd_page_offsets.argtypes = [P, P]
+        L.hdd_set_channel.argtypes = [P, I, P]
+        L.hdd_get_channel.argtypes = [P, I, P]
+        L.hdd_level_pages.argtypes = [P, I, C.POINTER(I)]
+        L.hdd_level_flags.argtypes = [P, I, P, P]
+        L.hdd_smooth.argtypes = [P, I]
+        L.hdd_vcycle.argtypes = [P]
+        L.hdd_apply_preconditioner.argtypes = [P]
+        L.hdd_sigma_info.argtypes = [P, C.POINTER(I), C.POINTER(I)]
+        L.hdd_interface_dofs.argtypes = [P, I, C.POINTER(I)]
+        L.hdd_interface_coords.argtypes = [P, I, P]
+        L.hdd_interface_matrix.argtypes = [P, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), P, P, P]
+        L.hdd_sigma_vcycle.argtypes = [P, P, P]
+        L.hdd_sigma_smooth.argtypes = [P, I, P, P]
+        L.hdd_pcg_solve.argtypes = [P, C.POINTER(PcgParams), C.POINTER(PcgReport), P, I]
+        L.hdd_setup_seconds.argtypes = [P, C.POINTER(C.c_double)]
+        for name in EXPORTS:
+            if name != "hdd_last_error":
+                getattr(L, name).restype = I
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DDSolver:
+    """Host facade over one hdd_ctx (Read_SPGrid + Initialize_Preconditioner in the constructor)."""
+
+    def __init__(self, dom, cfg):
+        L = lib()
+        self._keep = [np.ascontiguousarray(dom.page_offsets, dtype=np.uint64),
+                      np.ascontiguousarray(dom.flags, dtype=np.uint32),
+                      np.ascontiguousarray(dom.b, dtype=np.float32)]
+        d = ProblemDesc()
+        for a in range(3):
+            d.size[a] = int(dom.size[a])
+            d.sub_size[a] = int(cfg.sub_size)
+        d.vlevels, d.sigma_levels, d.nVcycles = cfg.vlevels, cfg.sigma_levels, cfg.nVcycles
+        d.interior_smoothing, d.boundary_smoothing = cfg.interior_smoothing, cfg.boundary_smoothing
+        d.sigma_smoothing, d.sigma_mg = cfg.sigma_smoothing, cfg.sigma_mg
+        d.n_pages = self._keep[0].shape[0]
+        d.page_offsets, d.flags, d.b = (_ptr(a) for a in self._keep)
+        h = C.c_void_p()
+        st = L.hdd_create(C.byref(d), C.byref(h))
+        if st != 0:
+            raise HddError(f"hdd_create failed ({st}): {L.hdd_last_error(None).decode()}")
+        self.h = h
+        n = C.c_int()
+        self._check(L.hdd_num_pages(self.h, C.byref(n)))
+        self.n_pages = n.value
+
+    def _check(self, st, ok=(0,)):
+        if st not in ok:
+            raise HddError(f"hdd call failed ({st}): {lib().hdd_last_error(self.h).decode()}")
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hdd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def setup_seconds(self):
+        s = C.c_double()
+        self._check(lib().hdd_setup_seconds(self.h, C.byref(s)))
+        return s.value
+
+    def page_offsets(self):
+        out = np.zeros(self.n_pages, dtype=np.uint64)
+        self._check(lib().hdd_page_offsets(self.h, _ptr(out)))
+        return out
+
+    def set(self, ch, v):
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(self.n_pages, 256)
+        self._check(lib().hdd_set_channel(self.h, ch, _ptr(v)))
+
+    def get(self, ch):
+        out = np.zeros((self.n_pages, 256), dtype=np.float32)
+        self._check(lib().hdd_get_channel(self.h, ch, _ptr(out)))
+        return out
+
+    def level(self, l):
+        n = C.c_int()
+        self._check(lib().hdd_level_pages(self.h, l, C.byref(n)))
+        off = np.zeros(n.value, dtype=np.uint64)
+        fl = np.zeros((n.value, 256), dtype=np.uint32)
+        self._check(lib().hdd_level_flags(self.h, l, _ptr(off), _ptr(fl)))
+        return off, fl
+
+    def smooth(self, mode):
+        self._check(lib().hdd_smooth(self.h, mode))
+
+    def vcycle(self, b):
+        self.set(CH_R, b)
+        self._check(lib().hdd_vcycle(self.h))
+        return self.get(CH_Z)
+
+    def apply_preconditioner(self, r):
+        self.set(CH_R, r)
+        self._check(lib().hdd_apply_preconditioner(self.h))
+        return self.get(CH_Z)
+
+    def sigma_info(self):
+        lv, ns = C.c_int(), C.c_int()
+        self._check(lib().hdd_sigma_info(self.h, C.byref(lv), C.byref(ns)))
+        return lv.value, ns.value
+
+    def interface_dofs(self, l):
+        n = C.c_int()
+        self._check(lib().hdd_interface_dofs(self.h, l, C.byref(n)))
+        return n.value
+
+    def interface_coords(self, l):
+        out = np.zeros((self.interface_dofs(l), 3), dtype=np.int32)
+        self._check(lib().hdd_interface_coords(self.h, l, _ptr(out)))
+        return out
+
+    def matrix(self, which, level, sub=0):
+        w = {"Arr": 0, "Aii": 1, "Air": 2, "Prr": 3}[which]
+        r, c, nz = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().hdd_interface_matrix(self.h, w, level, sub, C.byref(r), C.byref(c), C.byref(nz), None, None, None))
+        ptr = np.zeros(r.value + 1, dtype=np.int32)
+        idx = np.zeros(nz.value, dtype=np.int32)
+        val = np.zeros(nz.value, dtype=np.float32)
+        self._check(lib().hdd_interface_matrix(self.h, w, level, sub, C.byref(r), C.byref(c), C.byref(nz),
+                                               _ptr(ptr), _ptr(idx), _ptr(val)))
+        return r.value, c.value, ptr, idx, val
+
+    def sigma_vcycle(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        u = np.zeros_like(b)
+        self._check(lib().hdd_sigma_vcycle(self.h, _ptr(b), _ptr(u)))
+        return u
+
+    def sigma_smooth(self, level, b, u):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        u = np.ascontiguousarray(u, dtype=np.float32).copy()
+        self._check(lib().hdd_sigma_smooth(self.h, level, _ptr(b), _ptr(u)))
+        return u
+
+    def solve(self, tolerance: float, max_iterations: int = 5000):
+        """PCG from x=0; returns (iterations, residual history, seconds)."""
+        p = PcgParams(tolerance, max_iterations)
+        rep = PcgReport()
+        hist = np.zeros(max_iterations + 1, dtype=np.float64)
+        self._check(lib().hdd_pcg_solve(self.h, C.byref(p), C.byref(rep), _ptr(hist), max_iterations + 1), ok=(0, 4))
+        n = abs(rep.iterations) + 1
+        return rep.iterations, hist[:n], rep.seconds
+
+    # reference-facade spellings
+    Solve = solve

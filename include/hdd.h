@@ -1,0 +1,141 @@
+/*
+ * hdd.h — C ABI of the MI355X hierarchical domain-decomposition PCG solver.
+ *
+ * The library stands in for the reference's Domain_Decomposition PCG path
+ * (demos/Domain_Decomposition_CPU_Only). The reference has no C ABI; its
+ * interface is the C++ solver facade and the Krylov plugin classes. Each entry
+ * point below names the reference function it replaces.
+ *
+ * Conventions: plain pointers and sizes only; every call returns hdd_status
+ * (0 = success) and never throws; hdd_last_error() returns the message of the
+ * last failing call on a context (or of hdd_create when ctx is NULL). One
+ * context per host thread; calls on one context are serialised on the
+ * context's HIP stream. All device memory is owned by the context.
+ *
+ * Data layout: SPGrid pages of 4(x) x 8(y) x 8(z) cells (z fastest inside a
+ * page). `page_offsets` are SPGrid linear offsets of the pages (Morton page
+ * index << 12), as stored in the reference's `block_offset` file
+ * (DDS/SPGRID_READ_WRITE.h:95-106); `flags` and `b` hold 256 values per page
+ * in page-local order, as in its `flags` and `data` files.
+ */
+#ifndef HDD_H
+#define HDD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int hdd_status;
+enum { HDD_OK = 0, HDD_ERR_ARG = 1, HDD_ERR_SETUP = 2, HDD_ERR_DEVICE = 3, HDD_ERR_NOT_CONVERGED = 4 };
+
+typedef struct hdd_ctx hdd_ctx;
+
+/* Problem description: the reference's PARSE_ARGS flags
+ * (CPU/Test_Bench_3D/main.cpp:235-252) plus the SPGrid input arrays. */
+typedef struct hdd_problem_desc {
+    int size[3];            /* -size (solver size = generator size + 1) */
+    int sub_size[3];        /* -sub_size */
+    int vlevels;            /* -vlevels */
+    int sigma_levels;       /* -sigma_levels */
+    int nVcycles;           /* -nVcycles */
+    int interior_smoothing; /* -interior_smoothing */
+    int boundary_smoothing; /* -boundary_smoothing */
+    int sigma_smoothing;    /* -sigma_smoothing */
+    int sigma_mg;           /* -sigma_mg */
+    int n_pages;
+    const uint64_t* page_offsets; /* [n_pages] */
+    const uint32_t* flags;        /* [n_pages*256] */
+    const float* b;               /* [n_pages*256] right-hand side */
+} hdd_problem_desc;
+
+typedef struct hdd_pcg_params {
+    double tolerance;  /* stop when max|r| <= tolerance (absolute, L-infinity) */
+    int max_iterations;
+} hdd_pcg_params;
+
+typedef struct hdd_pcg_report {
+    int iterations;        /* negative if not converged within max_iterations */
+    double seconds;        /* wall time of the solve on the context stream */
+    double final_norm;     /* max|r| at exit */
+} hdd_pcg_report;
+
+/* Replaces SPGrid_Domain_Decomposition_Solver::Read_SPGrid + Initialize_Preconditioner
+ * (DDS/SPGrid_Domain_Decomposition_Solver.h:73-83, :121-161): reads the pages,
+ * splits subdomains, builds the interface operators and their factors, the
+ * V-cycle topology and flags, and uploads everything to the current device. */
+hdd_status hdd_create(const hdd_problem_desc* desc, hdd_ctx** out);
+
+/* Releases all device and host memory of a context. */
+hdd_status hdd_destroy(hdd_ctx* ctx);
+
+/* Message of the last failure (ctx may be NULL for hdd_create failures). */
+const char* hdd_last_error(const hdd_ctx* ctx);
+
+/* Number of level-0 pages (page order = Morton order of page_offsets). */
+hdd_status hdd_num_pages(const hdd_ctx* ctx, int* n_pages);
+
+/* Copies the context's level-0 page offsets (Morton order) into out[n_pages]. */
+hdd_status hdd_page_offsets(const hdd_ctx* ctx, uint64_t* out);
+
+/* Channel ids of the level-0 grid, matching the reference's field assignment
+ * in Read_SPGrid (DDS/SPGrid_Domain_Decomposition_Solver.h:75-82). */
+enum { HDD_CH_Z = 0, HDD_CH_R = 1, HDD_CH_K = 2, HDD_CH_X = 3, HDD_CH_B = 4, HDD_CH_Q = 5, HDD_CH_S = 6 };
+
+/* Host <-> device copies of a level-0 channel, n_pages*256 floats in page order. */
+hdd_status hdd_set_channel(hdd_ctx* ctx, int channel, const float* host);
+hdd_status hdd_get_channel(hdd_ctx* ctx, int channel, float* host);
+
+/* Level-l flags after setup (n = pages of level l, out[n*256]); pages of
+ * level l in Morton order, offsets into offsets_out[n] when not NULL. */
+hdd_status hdd_level_pages(const hdd_ctx* ctx, int level, int* n_pages);
+hdd_status hdd_level_flags(const hdd_ctx* ctx, int level, uint64_t* offsets_out, uint32_t* flags_out);
+
+/* One damped-Jacobi sweep on level 0 of z (channel 0) with rhs r (channel 1):
+ * mode 0 interior, 1 boundary, 2 all Active. Replaces V_Cycle_Helper::
+ * {interior,boundary,bottom}_smoothing (LIB/SPGrid_V_Cycle_Helper.h:69-112). */
+hdd_status hdd_smooth(hdd_ctx* ctx, int mode);
+
+/* z = 0, then one V-cycle for z from r. Replaces V_Cycle_Helper::v_cycle
+ * (LIB/SPGrid_V_Cycle_Helper.h:156-192). */
+hdd_status hdd_vcycle(hdd_ctx* ctx);
+
+/* z = M^{-1} r followed by Project(z). Replaces SPGrid_Domain_Decomposition_
+ * Preconditioner::Apply_Preconditioner (DDS/SPGrid_Domain_Decomposition_Preconditioner.h:88-128)
+ * as called through KRYLOV_SYSTEM_BASE::Precondition. */
+hdd_status hdd_apply_preconditioner(hdd_ctx* ctx);
+
+/* Interface solver introspection: sigma levels, subdomains, interface dofs. */
+hdd_status hdd_sigma_info(const hdd_ctx* ctx, int* levels, int* n_subdomains);
+hdd_status hdd_interface_dofs(const hdd_ctx* ctx, int level, int* n);
+hdd_status hdd_interface_coords(const hdd_ctx* ctx, int level, int* xyz);
+
+/* Interface operator export (which: 0 Arr, 1 Aii, 2 Air, 3 Prr) as CSR with
+ * sorted columns; first call with ptr=NULL to get rows/cols/nnz. Replaces the
+ * matrices of INTERFACE_MATRIX_GENERATOR::Convert_To_Eigen
+ * (IFS/INTERFACE_MATRIX_GENERATOR.h:660-734). */
+hdd_status hdd_interface_matrix(const hdd_ctx* ctx, int which, int level, int subdomain,
+                                int* rows, int* cols, int* nnz, int* ptr, int* idx, float* val);
+
+/* One sigma multigrid V-cycle on a level-0 interface vector: u = V(b), u0 = 0.
+ * Replaces SIGMA_MULTIGRID_SOLVER::V_Cycle (IFS/SIGMA_MULTIGRID_SOLVER.cpp:36-47). */
+hdd_status hdd_sigma_vcycle(hdd_ctx* ctx, const float* b_host, float* u_host);
+
+/* One sigma smoothing step on level l: u <- Smooth(u, b). Replaces
+ * SIGMA_SMOOTHER::Smooth (IFS/SIGMA_SMOOTHER.cpp:292-324). */
+hdd_status hdd_sigma_smooth(hdd_ctx* ctx, int level, const float* b_host, float* u_host);
+
+/* Preconditioned CG from x = 0 on the context's b. Replaces
+ * SPGrid_Domain_Decomposition_Solver::Solve -> CONJUGATE_GRADIENT_DUMPING::Solve
+ * (DDS/SPGrid_Domain_Decomposition_Solver.h:165-197, DDS/CONJUGATE_GRADIENT_DUMPING.cpp:146-212).
+ * residual_history (may be NULL) receives max|r| per iteration, up to history_len. */
+hdd_status hdd_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* params, hdd_pcg_report* report,
+                         double* residual_history, int history_len);
+
+/* Setup wall time in seconds. */
+hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

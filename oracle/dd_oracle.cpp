@@ -274,7 +274,7 @@ void ora_get_channel(void* hp, int c, double* out) {
 }
 void ora_set_channel(void* hp, int c, const double* in) {
     auto* h = (OraHandle*)hp;
-    ORA_DISPATCH(h, ([&] { size_t n = 0; for (int p : s.g.blocks) for (int k = 0; k < BCELLS; ++k) s.g.ch[c][(size_t)p * BCELLS + k] = (decltype(s.g.ch[0][0]))in[n++]; return 0; })());
+    ORA_DISPATCH(h, ([&] { size_t n = 0; for (int p : s.g.blocks) for (int k = 0; k < BCELLS; ++k) s.g.ch[c][(size_t)p * BCELLS + k] = (typename std::remove_reference<decltype(s.g.ch[0][0])>::type)in[n++]; return 0; })());
 }
 
 // one V-cycle on the main grid: u=z(ch0) from b=r(ch1), tmp k(ch2); u and k are cleared first
