@@ -18,18 +18,13 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libhdd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "--offload-arch=gfx950",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", os.path.join(HERE, "..", "include")]
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC)
-                  if f.endswith((".cpp", ".hip")) and not f.endswith("_stub.cpp") or f == "hdd_sigma_stub.cpp" and _use_stub())
-
-
-def _use_stub():
-    return not os.path.exists(os.path.join(CSRC, "hdd_sigma_device.hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".cpp", ".hip")))
 
 
 def _compile(src):
@@ -48,7 +43,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", "--offload-arch=gfx950", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])

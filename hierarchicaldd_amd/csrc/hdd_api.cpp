@@ -25,7 +25,7 @@ static thread_local std::string g_create_error;
 
 extern "C" {
 
-hdd_status hdd_create(const hdd_problem_desc* d, hdd_ctx** out) {
+static hdd_status create_impl(const hdd_problem_desc* d, hdd_ctx** out, bool host_only) {
     if (!d || !out || d->n_pages < 0 || (d->n_pages > 0 && (!d->page_offsets || !d->flags || !d->b))) {
         g_create_error = "hdd_create: invalid arguments";
         return HDD_ERR_ARG;
@@ -46,7 +46,7 @@ hdd_status hdd_create(const hdd_problem_desc* d, hdd_ctx** out) {
         p.sig_s = d->sigma_smoothing; p.sig_mg = d->sigma_mg;
         auto* c = new hdd_ctx();
         try {
-            c->solver.reset(new Solver(p, d->n_pages, d->page_offsets, d->flags, d->b));
+            c->solver.reset(new Solver(p, d->n_pages, d->page_offsets, d->flags, d->b, host_only));
         } catch (...) { delete c; throw; }
         *out = c;
         return HDD_OK;
@@ -55,6 +55,9 @@ hdd_status hdd_create(const hdd_problem_desc* d, hdd_ctx** out) {
         return HDD_ERR_SETUP;
     }
 }
+
+hdd_status hdd_create(const hdd_problem_desc* d, hdd_ctx** out) { return create_impl(d, out, false); }
+hdd_status hdd_create_host_only(const hdd_problem_desc* d, hdd_ctx** out) { return create_impl(d, out, true); }
 
 hdd_status hdd_destroy(hdd_ctx* ctx) {
     if (!ctx) return HDD_ERR_ARG;
@@ -78,6 +81,7 @@ hdd_status hdd_page_offsets(const hdd_ctx* ctx, uint64_t* out) {
 }
 
 hdd_status hdd_set_channel(hdd_ctx* ctx, int ch, const float* host) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || !host || ch < 0 || ch > 6) return HDD_ERR_ARG;
     HDD_TRY(ctx, {
         Solver& S = *ctx->solver;
@@ -87,6 +91,7 @@ hdd_status hdd_set_channel(hdd_ctx* ctx, int ch, const float* host) {
 }
 
 hdd_status hdd_get_channel(hdd_ctx* ctx, int ch, float* host) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || !host || ch < 0 || ch > 6) return HDD_ERR_ARG;
     HDD_TRY(ctx, {
         Solver& S = *ctx->solver;
@@ -113,11 +118,13 @@ hdd_status hdd_level_flags(const hdd_ctx* ctx, int level, uint64_t* offsets, uin
 }
 
 hdd_status hdd_smooth(hdd_ctx* ctx, int mode) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || mode < 0 || mode > 2) return HDD_ERR_ARG;
     HDD_TRY(ctx, { ctx->solver->smooth0(mode); hdd_check(hipStreamSynchronize(ctx->solver->stream), "smooth"); })
 }
 
 hdd_status hdd_vcycle(hdd_ctx* ctx) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx) return HDD_ERR_ARG;
     HDD_TRY(ctx, {
         Solver& S = *ctx->solver;
@@ -128,6 +135,7 @@ hdd_status hdd_vcycle(hdd_ctx* ctx) {
 }
 
 hdd_status hdd_apply_preconditioner(hdd_ctx* ctx) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx) return HDD_ERR_ARG;
     HDD_TRY(ctx, { ctx->solver->apply_preconditioner(); hdd_check(hipStreamSynchronize(ctx->solver->stream), "apply"); })
 }
@@ -169,16 +177,19 @@ hdd_status hdd_interface_matrix(const hdd_ctx* ctx, int which, int level, int sd
 }
 
 hdd_status hdd_sigma_vcycle(hdd_ctx* ctx, const float* b, float* u) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || !b || !u) return HDD_ERR_ARG;
     HDD_TRY(ctx, { sigma_vcycle_host(*ctx->solver->sigma, b, u, ctx->solver->stream); })
 }
 
 hdd_status hdd_sigma_smooth(hdd_ctx* ctx, int level, const float* b, float* u) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || !b || !u || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
     HDD_TRY(ctx, { sigma_smooth_host(*ctx->solver->sigma, level, b, u, ctx->solver->stream); })
 }
 
 hdd_status hdd_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_report* rep, double* hist, int hist_len) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || !prm || prm->max_iterations < 0) return HDD_ERR_ARG;
     try {
         std::vector<double> h;

@@ -297,3 +297,42 @@ void launch_maxabs(const float* a, size_t n, float* part, float* scal, hipStream
 }
 
 }  // namespace hdd
+
+namespace hdd {
+
+// b0[i] = r_e - (float)(-sum over Active face neighbours of z, double accumulation)
+// for each interface dof i at level-0 cell e (Laplace_ri + Copy_To_Eigen_Array,
+// LIB/SPGrid_Laplace_Legacy.h:95-116, IFS/EIGEN_INTERFACE.cpp:18-30)
+__global__ void k_if_gather(DevLevel L, const int* __restrict__ if_index, int n, const float* __restrict__ z,
+                            const float* __restrict__ r, float* __restrict__ b0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int e = if_index[i];
+    if (e < 0) { b0[i] = 0.f; return; }
+    const int p = e >> 8, c = e & 255;
+    const uint32_t f = L.flags[e];
+    double res = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if (f & (kFaceMX << k)) {
+            const int q = nb_index(L.nbr, p, c, k);
+            if (L.flags[q] & kActive) res -= (double)z[q];
+        }
+    b0[i] = r[e] - (float)res;
+}
+
+__global__ void k_if_scatter(const int* __restrict__ if_index, int n, const float* __restrict__ u0, float* __restrict__ z) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int e = if_index[i];
+    if (e >= 0) z[e] = u0[i];
+}
+
+void launch_if_gather(const DevLevel& L0, const int* if_index, int n, const float* z, const float* r, float* b0, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_if_gather, dim3((n + 255) / 256), dim3(256), 0, s, L0, if_index, n, z, r, b0);
+}
+void launch_if_scatter(const int* if_index, int n, const float* u0, float* z, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_if_scatter, dim3((n + 255) / 256), dim3(256), 0, s, if_index, n, u0, z);
+}
+
+}  // namespace hdd

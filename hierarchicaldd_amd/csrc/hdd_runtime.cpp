@@ -27,11 +27,10 @@ void Solver::build_levels(int n_pages, const uint64_t* page_offsets, const uint3
     L0.sort_pages();
 }
 
-Solver::Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b) : P(p) {
+Solver::Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b, bool host)
+    : P(p), host_only(host) {
     auto t0 = std::chrono::steady_clock::now();
     if (P.vlevels < 1) throw std::runtime_error("vlevels must be >= 1");
-    hdd_check(hipStreamCreate(&stream), "hipStreamCreate");
-    upload_constants();
     build_levels(n_pages, page_offsets, flags);
     // the interface operators are generated from the input flags, before division
     // (Initialize_Preconditioner: Split precedes Domain_Divider, DDS/...Solver.h:122-133)
@@ -47,6 +46,12 @@ Solver::Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets,
         set_face_flags(lv[l].host);
         set_boundary_flags(lv[l].host);
     }
+    if (host_only) {
+        setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return;
+    }
+    hdd_check(hipStreamCreate(&stream), "hipStreamCreate");
+    upload_constants();
     upload_levels();
     // rhs b in page order of level 0 (input order may differ: remap through the LUT)
     {

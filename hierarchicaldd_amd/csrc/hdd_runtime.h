@@ -61,7 +61,10 @@ public:
     hipStream_t stream = nullptr;
     double setup_seconds = 0;
 
-    Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b);
+    // host_only: run the host setup and skip every device allocation (introspection)
+    Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b,
+           bool host_only = false);
+    bool host_only = false;
     ~Solver();
 
     // level-0 channel access in page order (c: 0=z 1=r 2=k 3=x 4=b 5=q 6=s)

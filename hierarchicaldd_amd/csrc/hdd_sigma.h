@@ -25,6 +25,7 @@ struct SigmaHost {
     std::vector<std::vector<Int3>> if_coords;   // per level, level coordinates
     std::vector<Csr> Arr, Prr;
     std::vector<std::vector<Csr>> Aii, Air;     // [level][subdomain]
+    std::vector<std::vector<std::vector<Int3>>> Aii_coords;  // [level][subdomain][dof], fine coordinates
 };
 
 void generate_interface_operators(const struct PagedLevel& input_flags, Int3 size, Int3 sub, int sigma_levels, SigmaHost& out);

@@ -1,0 +1,375 @@
+// Sigma (interface Schur complement) multigrid on the GPU.
+//
+//   SIGMA_MULTIGRID_SOLVER::V_Cycle   IFS/SIGMA_MULTIGRID_SOLVER.cpp:36-47
+//   SIGMA_SMOOTHER::Smooth            IFS/SIGMA_SMOOTHER.cpp:292-324
+//   SIGMA_SMOOTHER::Compute_Residual  IFS/SIGMA_SMOOTHER.cpp:328-352
+//
+// Exact solves with (A + 1e-5 I) use supernodal Cholesky factors (host,
+// double) turned into float "triangular-solve plans": per supernode the
+// explicit inverse of its diagonal block and its dense sub-diagonal block,
+// plus a row-wise CSR of the sub-diagonal part for the forward gather. A solve
+// walks the supernode tree by height (leaves first forward, root first
+// backward); every row of a height level is one 64-lane wavefront, so a level
+// is two launches: a gather of the off-diagonal products and a dense product
+// with the diagonal-block inverse. All subdomains' Aii of one sigma level form
+// one batch, so each level of all their trees is a single launch.
+//
+// Sparse products keep Eigen's accumulation order (rows in column order, the
+// transposed products in source-row order), built as gathers so no atomics
+// are needed and results are deterministic.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <memory>
+#include <chrono>
+#include <omp.h>
+#include "hdd_factor.h"
+#include "hdd_runtime.h"
+#include "hdd_sigma_device.h"
+
+namespace hdd {
+
+namespace {
+
+__global__ void __launch_bounds__(256) k_tri_fwd_gather(const int* __restrict__ rows, int nrows, const int* __restrict__ orig_of,
+                                                        const float* __restrict__ b, const int* __restrict__ lp, const int* __restrict__ li,
+                                                        const float* __restrict__ lv, const float* __restrict__ y, float* __restrict__ t) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nrows) return;
+    const int i = rows[w];
+    float s = 0.f;
+    for (int e = lp[i] + lane; e < lp[i + 1]; e += 64) s += lv[e] * y[li[e]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) t[i] = b[orig_of[i]] - s;
+}
+
+__global__ void __launch_bounds__(256) k_tri_fwd_diag(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
+                                                      const int* __restrict__ s0, const int* __restrict__ sz, const long long* __restrict__ doff,
+                                                      const float* __restrict__ diag, const float* __restrict__ t, float* __restrict__ y) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nrows) return;
+    const int i = rows[w], s = sn_of[i], base = s0[s], n = sz[s], r = i - base;
+    const float* D = diag + doff[s] + (long long)r * n;
+    float acc = 0.f;
+    for (int k = lane; k <= r; k += 64) acc += D[k] * t[base + k];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) y[i] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_tri_bwd_gather(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
+                                                        const int* __restrict__ s0, const int* __restrict__ nr, const long long* __restrict__ loff,
+                                                        const int* __restrict__ roff, const int* __restrict__ srows, const float* __restrict__ l21,
+                                                        const float* __restrict__ y, float* __restrict__ t) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nrows) return;
+    const int i = rows[w], s = sn_of[i], r = i - s0[s], m = nr[s];
+    const float* Lc = l21 + loff[s] + (long long)r * m;
+    const int* R = srows + roff[s];
+    float acc = 0.f;
+    for (int a = lane; a < m; a += 64) acc += Lc[a] * y[R[a]];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) t[i] = y[i] - acc;
+}
+
+__global__ void __launch_bounds__(256) k_tri_bwd_diag(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
+                                                      const int* __restrict__ s0, const int* __restrict__ sz, const long long* __restrict__ doff,
+                                                      const float* __restrict__ diagT, const float* __restrict__ t, float* __restrict__ y,
+                                                      const int* __restrict__ orig_of, float* __restrict__ out) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= nrows) return;
+    const int i = rows[w], s = sn_of[i], base = s0[s], n = sz[s], r = i - base;
+    const float* D = diagT + doff[s] + (long long)r * n;
+    float acc = 0.f;
+    for (int k = r + lane; k < n; k += 64) acc += D[k] * t[base + k];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) { y[i] = acc; out[orig_of[i]] = acc; }
+}
+
+// y = A x for CSR A (one thread per row, column order)
+__global__ void k_csr_mv(int rows, const int* __restrict__ p, const int* __restrict__ ix, const float* __restrict__ v,
+                         const float* __restrict__ x, float* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    float acc = 0.f;
+    for (int e = p[i]; e < p[i + 1]; ++e) acc += v[e] * x[ix[e]];
+    y[i] = acc;
+}
+
+// out = b + sum (transposed product as a gather), optionally - A u (Compute_Residual)
+__global__ void k_schur_rhs(int rows, const float* __restrict__ b, const int* __restrict__ tp, const int* __restrict__ ti,
+                            const float* __restrict__ tv, const float* __restrict__ vx, const int* __restrict__ ap,
+                            const int* __restrict__ ai, const float* __restrict__ av, const float* __restrict__ u, float* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= rows) return;
+    float acc = b[c];
+    for (int e = tp[c]; e < tp[c + 1]; ++e) acc += tv[e] * vx[ti[e]];
+    if (ap)
+        for (int e = ap[c]; e < ap[c + 1]; ++e) acc += av[e] * (-u[ai[e]]);
+    out[c] = acc;
+}
+
+// u[i] += sum_c P(i,c) uc[c]
+__global__ void k_prolong_add(int rows, const int* __restrict__ p, const int* __restrict__ ix, const float* __restrict__ v,
+                              const float* __restrict__ uc, float* __restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    float acc = 0.f;
+    for (int e = p[i]; e < p[i + 1]; ++e) acc += v[e] * uc[ix[e]];
+    u[i] += acc;
+}
+
+}  // namespace
+
+static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
+
+void CsrDev::upload(const Csr& m) {
+    rows = m.rows; cols = m.cols;
+    p.upload(m.ptr); ix.upload(m.idx); v.upload(m.val);
+}
+
+static Csr transpose(const Csr& A) {
+    Csr T; T.rows = A.cols; T.cols = A.rows;
+    std::vector<int> cnt(A.cols + 1, 0);
+    for (int e = 0; e < (int)A.nnz(); ++e) cnt[A.idx[e] + 1]++;
+    for (int c = 0; c < A.cols; ++c) cnt[c + 1] += cnt[c];
+    T.ptr = cnt;
+    T.idx.resize(A.nnz()); T.val.resize(A.nnz());
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int r = 0; r < A.rows; ++r)  // source rows ascending -> each transposed row in source-row order
+        for (int e = A.ptr[r]; e < A.ptr[r + 1]; ++e) { const int k = fill[A.idx[e]]++; T.idx[k] = r; T.val[k] = A.val[e]; }
+    return T;
+}
+
+void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets) {
+    n = 0; H = -1;
+    for (size_t m = 0; m < F.size(); ++m) { n = std::max(n, offsets[m] + F[m]->n); H = std::max(H, F[m]->n ? F[m]->max_height : -1); }
+    if (n == 0) return;
+    std::vector<int> h_orig(n), h_snof(n);
+    std::vector<int> h_s0, h_sz, h_nr, h_roff, h_srows, h_height;
+    std::vector<long long> h_doff, h_loff;
+    long long dsize = 0, lsize = 0;
+    struct Ref { int m, s; };
+    std::vector<Ref> refs;
+    for (size_t m = 0; m < F.size(); ++m) {
+        const SupernodalFactor& f = *F[m];
+        const int off = offsets[m];
+        for (int k = 0; k < f.n; ++k) h_orig[off + k] = off + f.perm[k];
+        for (size_t s = 0; s < f.sn.size(); ++s) {
+            const Supernode& S = f.sn[s];
+            const int gid = (int)h_s0.size();
+            h_s0.push_back(off + S.s0); h_sz.push_back(S.n); h_nr.push_back((int)S.rows.size());
+            h_roff.push_back((int)h_srows.size());
+            for (int r : S.rows) h_srows.push_back(off + r);
+            h_doff.push_back(dsize); dsize += (long long)S.n * S.n;
+            h_loff.push_back(lsize); lsize += (long long)S.n * S.rows.size();
+            h_height.push_back(S.height);
+            for (int k = 0; k < S.n; ++k) h_snof[off + S.s0 + k] = gid;
+            refs.push_back({(int)m, (int)s});
+        }
+    }
+    const int nsn = (int)h_s0.size();
+    std::vector<float> h_diag(dsize), h_diagT(dsize), h_l21(lsize);
+    std::vector<int> lcount(n + 1, 0);
+    for (int g = 0; g < nsn; ++g)
+        for (int a = 0; a < h_nr[g]; ++a) lcount[h_srows[h_roff[g] + a] + 1] += h_sz[g];
+    for (int i = 0; i < n; ++i) lcount[i + 1] += lcount[i];
+    std::vector<int> h_li(lcount[n]);
+    std::vector<float> h_lv(lcount[n]);
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 0; g < nsn; ++g) {
+        const SupernodalFactor& f = *F[refs[g].m];
+        const int s = refs[g].s, ns = h_sz[g], nr = h_nr[g];
+        std::vector<double> inv(f.L11[s]);
+        invert_lower(inv, ns);
+        float* D = &h_diag[h_doff[g]];
+        float* DT = &h_diagT[h_doff[g]];
+        for (int r = 0; r < ns; ++r)
+            for (int k = 0; k < ns; ++k) {
+                D[(size_t)r * ns + k] = k <= r ? (float)inv[(size_t)k * ns + r] : 0.f;   // Linv(r,k)
+                DT[(size_t)r * ns + k] = k >= r ? (float)inv[(size_t)r * ns + k] : 0.f;  // Linv(k,r)
+            }
+        float* L = &h_l21[h_loff[g]];
+        for (size_t e = 0; e < (size_t)ns * nr; ++e) L[e] = (float)f.L21[s][e];
+    }
+    // forward gather rows: supernodes in order give ascending columns per row
+    {
+        std::vector<int> fill(lcount.begin(), lcount.end() - 1);
+        for (int g = 0; g < nsn; ++g) {
+            const int ns = h_sz[g], nr = h_nr[g];
+            for (int j = 0; j < ns; ++j)
+                for (int a = 0; a < nr; ++a) {
+                    const int row = h_srows[h_roff[g] + a];
+                    const int k = fill[row]++;
+                    h_li[k] = h_s0[g] + j;
+                    h_lv[k] = h_l21[h_loff[g] + (size_t)j * nr + a];
+                }
+        }
+    }
+    std::vector<int> h_rows;
+    lev_begin.assign(H + 2, 0);
+    for (int h = 0; h <= H; ++h) {
+        lev_begin[h] = (int)h_rows.size();
+        for (int g = 0; g < nsn; ++g)
+            if (h_height[g] == h) for (int k = 0; k < h_sz[g]; ++k) h_rows.push_back(h_s0[g] + k);
+    }
+    lev_begin[H + 1] = (int)h_rows.size();
+    orig_of.upload(h_orig); sn_of.upload(h_snof); s0.upload(h_s0); sz.upload(h_sz); nr.upload(h_nr);
+    roff.upload(h_roff); srows.upload(h_srows); rowlist.upload(h_rows); lp.upload(lcount); li.upload(h_li);
+    doff.upload(h_doff); loff.upload(h_loff);
+    diag.upload(h_diag); diagT.upload(h_diagT); l21.upload(h_l21); lv.upload(h_lv);
+    y.alloc(n); t.alloc(n);
+}
+
+size_t TriPlan::bytes() const {
+    return (orig_of.n + sn_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + rowlist.n + lp.n + li.n) * 4 + (doff.n + loff.n) * 8 +
+           (diag.n + diagT.n + l21.n + lv.n + y.n + t.n) * 4;
+}
+
+void TriPlan::solve(const float* b, float* x, hipStream_t s) {
+    if (n == 0) return;
+    for (int h = 0; h <= H; ++h) {
+        const int cnt = lev_begin[h + 1] - lev_begin[h];
+        if (!cnt) continue;
+        const int* rows = rowlist.p + lev_begin[h];
+        hipLaunchKernelGGL(k_tri_fwd_gather, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, orig_of.p, b, lp.p, li.p, lv.p, y.p, t.p);
+        hipLaunchKernelGGL(k_tri_fwd_diag, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diag.p, t.p, y.p);
+    }
+    for (int h = H; h >= 0; --h) {
+        const int cnt = lev_begin[h + 1] - lev_begin[h];
+        if (!cnt) continue;
+        const int* rows = rowlist.p + lev_begin[h];
+        hipLaunchKernelGGL(k_tri_bwd_gather, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, sn_of.p, s0.p, nr.p, loff.p, roff.p, srows.p, l21.p, y.p, t.p);
+        hipLaunchKernelGGL(k_tri_bwd_diag, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diagT.p, t.p, y.p, orig_of.p, x);
+    }
+}
+
+SigmaDevice::~SigmaDevice() {}
+
+std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing, int n_mg) {
+    std::unique_ptr<SigmaDevice> S(new SigmaDevice());
+    S->levels = H.levels; S->smoothing = smoothing; S->n_mg = n_mg;
+    S->lv.resize(H.levels);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int l = 0; l < H.levels; ++l) {
+        SigmaLevel& L = S->lv[l];
+        L.n_if = H.n_if[l];
+        // concatenate the subdomains' interior systems
+        std::vector<int> off(H.nsub + 1, 0);
+        for (int s = 0; s < H.nsub; ++s) off[s + 1] = off[s] + H.Aii[l][s].rows;
+        L.n_int = off[H.nsub];
+        Csr air; air.rows = L.n_int; air.cols = L.n_if; air.ptr.assign(1, 0);
+        for (int s = 0; s < H.nsub; ++s) {
+            const Csr& A = H.Air[l][s];
+            for (int r = 0; r < A.rows; ++r) {
+                for (int e = A.ptr[r]; e < A.ptr[r + 1]; ++e) { air.idx.push_back(A.idx[e]); air.val.push_back(A.val[e]); }
+                air.ptr.push_back((int)air.idx.size());
+            }
+        }
+        L.Air.upload(air);
+        L.AirT.upload(transpose(air));
+        L.Arr.upload(H.Arr[l]);
+        if (l + 1 < H.levels) { L.P.upload(H.Prr[l]); L.PT.upload(transpose(H.Prr[l])); }
+        std::vector<SupernodalFactor> fi(H.nsub);
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < H.nsub; ++s)
+            if (H.Aii[l][s].rows > 0) factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, fi[s]);
+        std::vector<const SupernodalFactor*> fp;
+        std::vector<int> fo;
+        for (int s = 0; s < H.nsub; ++s) if (H.Aii[l][s].rows > 0) { fp.push_back(&fi[s]); fo.push_back(off[s]); }
+        L.Lii.build(fp, fo);
+        std::vector<SupernodalFactor>().swap(fi);
+        std::vector<Int3> rc(H.if_coords[l].size());
+        for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l};
+        SupernodalFactor fr;
+        factor_spd(H.Arr[l], rc, 1e-5, fr);
+        L.Lrr.build({&fr}, {0});
+        L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
+        L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
+        S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
+    }
+    S->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (H.levels > 0) { S->b0 = S->lv[0].b.p; S->u0 = S->lv[0].u.p; }
+    return S;
+}
+
+// tmp = b + sum_s Air_s^T Aii_s^{-1} Air_s u, then u = Arr^{-1} tmp
+void SigmaDevice::smooth(int l, hipStream_t s) {
+    SigmaLevel& L = lv[l];
+    if (L.n_if == 0) return;
+    if (L.n_int) {
+        hipLaunchKernelGGL(k_csr_mv, dim3(blocks_for(L.n_int, 256)), dim3(256), 0, s, L.n_int, L.Air.p.p, L.Air.ix.p, L.Air.v.p, L.u.p, L.vb.p);
+        L.Lii.solve(L.vb.p, L.vx.p, s);
+    }
+    hipLaunchKernelGGL(k_schur_rhs, dim3(blocks_for(L.n_if, 256)), dim3(256), 0, s, L.n_if, L.b.p, L.AirT.p.p, L.AirT.ix.p, L.AirT.v.p,
+                       (const float*)L.vx.p, (const int*)nullptr, (const int*)nullptr, (const float*)nullptr, (const float*)nullptr, L.tmp.p);
+    L.Lrr.solve(L.tmp.p, L.u.p, s);
+}
+
+// r = b + sum_s Air_s^T Aii_s^{-1} Air_s u - Arr u
+void SigmaDevice::residual(int l, hipStream_t s) {
+    SigmaLevel& L = lv[l];
+    if (L.n_if == 0) return;
+    if (L.n_int) {
+        hipLaunchKernelGGL(k_csr_mv, dim3(blocks_for(L.n_int, 256)), dim3(256), 0, s, L.n_int, L.Air.p.p, L.Air.ix.p, L.Air.v.p, L.u.p, L.vb.p);
+        L.Lii.solve(L.vb.p, L.vx.p, s);
+    }
+    hipLaunchKernelGGL(k_schur_rhs, dim3(blocks_for(L.n_if, 256)), dim3(256), 0, s, L.n_if, L.b.p, L.AirT.p.p, L.AirT.ix.p, L.AirT.v.p,
+                       (const float*)L.vx.p, (const int*)L.Arr.p.p, (const int*)L.Arr.ix.p, (const float*)L.Arr.v.p, (const float*)L.u.p, L.r.p);
+}
+
+void SigmaDevice::vcycle(hipStream_t s) {
+    const int NL = levels;
+    for (int l = 0; l + 1 < NL; ++l) {
+        for (int i = 0; i < smoothing; ++i) smooth(l, s);
+        residual(l, s);
+        SigmaLevel& C = lv[l + 1];
+        launch_fill(C.u.p, 0.f, C.u.n, s);
+        if (C.n_if)
+            hipLaunchKernelGGL(k_csr_mv, dim3(blocks_for(C.n_if, 256)), dim3(256), 0, s, C.n_if, lv[l].PT.p.p, lv[l].PT.ix.p, lv[l].PT.v.p,
+                               (const float*)lv[l].r.p, C.b.p);
+    }
+    for (int i = 0; i < 200; ++i) smooth(NL - 1, s);
+    for (int l = NL - 2; l >= 0; --l) {
+        SigmaLevel& F = lv[l];
+        if (F.n_if)
+            hipLaunchKernelGGL(k_prolong_add, dim3(blocks_for(F.n_if, 256)), dim3(256), 0, s, F.n_if, F.P.p.p, F.P.ix.p, F.P.v.p,
+                               (const float*)lv[l + 1].u.p, F.u.p);
+        for (int i = 0; i < smoothing; ++i) smooth(l, s);
+    }
+}
+
+void sigma_vcycles(SigmaDevice& S, hipStream_t s) {
+    if (S.levels == 0) return;
+    launch_fill(S.u0, 0.f, S.lv[0].u.n, s);
+    for (int i = 0; i < S.n_mg; ++i) S.vcycle(s);
+}
+
+void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s) {
+    if (S.levels == 0 || S.lv[0].n_if == 0) return;
+    launch_if_gather(L0, if_index, S.lv[0].n_if, z, r, S.b0, s);
+}
+
+void launch_interface_scatter(SigmaDevice& S, const int* if_index, float* z, hipStream_t s) {
+    if (S.levels == 0 || S.lv[0].n_if == 0) return;
+    launch_if_scatter(if_index, S.lv[0].n_if, S.u0, z, s);
+}
+
+void sigma_vcycle_host(SigmaDevice& S, const float* b, float* u, hipStream_t s) {
+    if (S.levels == 0) throw std::runtime_error("no sigma levels");
+    const size_t n = S.lv[0].n_if;
+    hdd_check(hipMemcpyAsync(S.b0, b, n * sizeof(float), hipMemcpyHostToDevice, s));
+    sigma_vcycles(S, s);
+    hdd_check(hipMemcpyAsync(u, S.u0, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    hdd_check(hipStreamSynchronize(s));
+}
+
+void sigma_smooth_host(SigmaDevice& S, int l, const float* b, float* u, hipStream_t s) {
+    SigmaLevel& L = S.lv.at(l);
+    const size_t n = L.n_if;
+    hdd_check(hipMemcpyAsync(L.b.p, b, n * sizeof(float), hipMemcpyHostToDevice, s));
+    hdd_check(hipMemcpyAsync(L.u.p, u, n * sizeof(float), hipMemcpyHostToDevice, s));
+    S.smooth(l, s);
+    hdd_check(hipMemcpyAsync(u, L.u.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    hdd_check(hipStreamSynchronize(s));
+}
+
+}  // namespace hdd

@@ -38,7 +38,7 @@ class PcgReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("final_norm", C.c_double)]
 
 
-EXPORTS = ["hdd_create", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
+EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
            "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
            "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds"]
@@ -53,6 +53,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P, I = C.c_void_p, C.c_int
         L.hdd_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
+        L.hdd_create_host_only.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
         L.hdd_destroy.argtypes = [P]
         L.hdd_last_error.argtypes = [P]
         L.hdd_last_error.restype = C.c_char_p
@@ -87,7 +88,7 @@ def _ptr(a):
 class DDSolver:
     """Host facade over one hdd_ctx (Read_SPGrid + Initialize_Preconditioner in the constructor)."""
 
-    def __init__(self, dom, cfg):
+    def __init__(self, dom, cfg, host_only: bool = False):
         L = lib()
         self._keep = [np.ascontiguousarray(dom.page_offsets, dtype=np.uint64),
                       np.ascontiguousarray(dom.flags, dtype=np.uint32),
@@ -102,7 +103,7 @@ class DDSolver:
         d.n_pages = self._keep[0].shape[0]
         d.page_offsets, d.flags, d.b = (_ptr(a) for a in self._keep)
         h = C.c_void_p()
-        st = L.hdd_create(C.byref(d), C.byref(h))
+        st = (L.hdd_create_host_only if host_only else L.hdd_create)(C.byref(d), C.byref(h))
         if st != 0:
             raise HddError(f"hdd_create failed ({st}): {L.hdd_last_error(None).decode()}")
         self.h = h

@@ -66,6 +66,11 @@ typedef struct hdd_pcg_report {
  * V-cycle topology and flags, and uploads everything to the current device. */
 hdd_status hdd_create(const hdd_problem_desc* desc, hdd_ctx** out);
 
+/* Same setup as hdd_create, host side only: no device memory is touched and
+ * every device entry point on the context returns HDD_ERR_ARG. For inspecting
+ * flags and interface operators on machines without a GPU. */
+hdd_status hdd_create_host_only(const hdd_problem_desc* desc, hdd_ctx** out);
+
 /* Releases all device and host memory of a context. */
 hdd_status hdd_destroy(hdd_ctx* ctx);
 

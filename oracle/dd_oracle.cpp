@@ -14,6 +14,7 @@
 #include "dd_sigma.h"
 #include "dd_vcycle.h"
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <memory>
 #include <string>
@@ -57,11 +58,18 @@ struct Solver {
     // Initialize_Preconditioner (DDS/SPGrid_Domain_Decomposition_Solver.h:121-161)
     void setup() {
         auto t0 = std::chrono::steady_clock::now();
+        const bool verbose = std::getenv("HDD_ORACLE_VERBOSE") != nullptr;
+        auto lap = [&](const char* what) {
+            if (verbose) std::fprintf(stderr, "[oracle] %s %.2fs\n", what, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        };
         split_and_generate(g, size, sub, S, P.sigma_levels);
+        lap("interface operators");
         domain_divider(g, size, sub);
         flaging(g);
         mg.initialize(S, P.sig_s);
+        lap("factorisation");
         build_topology(topo, g, size, P.vlevels);
+        lap("topology");
         t_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
 
