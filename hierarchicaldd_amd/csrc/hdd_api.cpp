@@ -14,9 +14,9 @@ struct hdd_ctx {
 
 static thread_local std::string g_create_error;
 
-#define HDD_TRY(ctx, body)                                  \
+#define HDD_TRY(ctx, ...)                                   \
     try {                                                   \
-        body;                                               \
+        __VA_ARGS__;                                        \
         return HDD_OK;                                      \
     } catch (std::exception & e) {                          \
         if (ctx) (ctx)->err = e.what();                     \
@@ -207,6 +207,36 @@ hdd_status hdd_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_report
 hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* s) {
     if (!ctx || !s) return HDD_ERR_ARG;
     *s = ctx->solver->setup_seconds;
+    return HDD_OK;
+}
+
+hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx || reps <= 0 || !avg_ms) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        Solver& S = *ctx->solver;
+        hipEvent_t a, b;
+        hdd_check(hipEventCreate(&a)); hdd_check(hipEventCreate(&b));
+        S.smooth0(2);  // warm
+        hdd_check(hipEventRecord(a, S.stream));
+        for (int i = 0; i < reps; ++i) S.smooth0(2);
+        hdd_check(hipEventRecord(b, S.stream));
+        hdd_check(hipEventSynchronize(b));
+        float ms = 0.f;
+        hdd_check(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+        *avg_ms = ms / reps;
+        if (bytes) *bytes = 16.0 * kPageCells * S.lv[0].host.npages();
+    })
+}
+
+hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
+    if (!ctx || !out) return HDD_ERR_ARG;
+    const Solver& S = *ctx->solver;
+    double v[7] = {S.setup_seconds, S.sigma ? S.sigma->factor_seconds : 0.0, S.sigma ? (double)S.sigma->plan_bytes : 0.0,
+                   (double)S.lv[0].host.npages(), S.sigma_host.levels ? (double)S.sigma_host.n_if[0] : 0.0,
+                   (S.sigma && S.sigma->levels) ? (double)S.sigma->lv[0].n_int : 0.0, (double)S.sigma_host.nsub};
+    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
     return HDD_OK;
 }
 

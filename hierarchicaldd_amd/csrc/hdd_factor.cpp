@@ -1,4 +1,5 @@
 #include "hdd_factor.h"
+#include "hdd_parallel.h"
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -255,12 +256,14 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
     for (int h = 0; h <= F.max_height; ++h) {
         const auto& list = by_h[h];
         if ((int)list.size() >= nthreads) {
+            ParallelErrors errs;
 #pragma omp parallel
             {
                 std::vector<int> pos(n);
 #pragma omp for schedule(dynamic)
-                for (size_t k = 0; k < list.size(); ++k) front(list[k], false, pos);
+                for (size_t k = 0; k < list.size(); ++k) errs.run([&] { front(list[k], false, pos); });
             }
+            errs.rethrow();
         } else {
             std::vector<int> pos(n);
             for (int s : list) front(s, true, pos);

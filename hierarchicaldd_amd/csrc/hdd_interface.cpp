@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <unordered_map>
+#include "hdd_parallel.h"
 #include "hdd_setup.h"
 #include "hdd_sigma.h"
 
@@ -489,16 +490,19 @@ void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, 
     const Int3 ns = subdomain_counts(size, sub);
     const int nsub = ns.x * ns.y * ns.z;
     std::vector<Subdomain> subs(nsub);
+    ParallelErrors errs;
     // phase 1 (parallel): split, activate, number local dofs
 #pragma omp parallel for schedule(dynamic)
-    for (int s = 0; s < nsub; ++s) {
-        const int sx = s / (ns.y * ns.z), sy = (s / ns.z) % ns.y, sz = s % ns.z;
-        Subdomain& S = subs[s];
-        S.lo = {sx * sub.x, sy * sub.y, sz * sub.z};
-        S.hi = {S.lo.x + sub.x, S.lo.y + sub.y, S.lo.z + sub.z};
-        split_subdomain(input, sub, S);
-        build_hierarchy(S, mgl);
-    }
+    for (int s = 0; s < nsub; ++s)
+        errs.run([&] {
+            const int sx = s / (ns.y * ns.z), sy = (s / ns.z) % ns.y, sz = s % ns.z;
+            Subdomain& S = subs[s];
+            S.lo = {sx * sub.x, sy * sub.y, sz * sub.z};
+            S.hi = {S.lo.x + sub.x, S.lo.y + sub.y, S.lo.z + sub.z};
+            split_subdomain(input, sub, S);
+            build_hierarchy(S, mgl);
+        });
+    errs.rethrow();
     // phase 2 (serial, subdomain order): global interface numbering per level
     out.levels = mgl; out.nsub = nsub;
     out.n_if.assign(mgl, 0);
@@ -517,7 +521,8 @@ void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, 
             }
     // phase 3 (parallel): Galerkin assembly per subdomain
 #pragma omp parallel for schedule(dynamic)
-    for (int s = 0; s < nsub; ++s) assemble(subs[s], mgl, if_id);
+    for (int s = 0; s < nsub; ++s) errs.run([&] { assemble(subs[s], mgl, if_id); });
+    errs.rethrow();
     // phase 4 (serial, subdomain order): accumulate Arr in the reference's order
     std::vector<std::vector<std::vector<std::pair<int, float>>>> arr(mgl);
     for (int l = 0; l < mgl; ++l) arr[l].assign(out.n_if[l], {});

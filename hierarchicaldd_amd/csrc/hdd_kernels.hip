@@ -59,8 +59,9 @@ __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restr
     const uint32_t sel = f & (kActive | kBoundary);
     const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
     float u = u_in[e];
+    const float bb = b[e];
     if (w) {
-        const float r = cell_residual(L.nbr, p, c, f, u_in, b[e]);
+        const float r = cell_residual(L.nbr, p, c, f, u_in, bb);
         u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * r;
     }
     u_out[e] = u;

@@ -23,6 +23,7 @@
 #include <chrono>
 #include <omp.h>
 #include "hdd_factor.h"
+#include "hdd_parallel.h"
 #include "hdd_runtime.h"
 #include "hdd_sigma_device.h"
 
@@ -269,9 +270,11 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing
         L.Arr.upload(H.Arr[l]);
         if (l + 1 < H.levels) { L.P.upload(H.Prr[l]); L.PT.upload(transpose(H.Prr[l])); }
         std::vector<SupernodalFactor> fi(H.nsub);
+        ParallelErrors errs;
 #pragma omp parallel for schedule(dynamic)
         for (int s = 0; s < H.nsub; ++s)
-            if (H.Aii[l][s].rows > 0) factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, fi[s]);
+            if (H.Aii[l][s].rows > 0) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, fi[s]); });
+        errs.rethrow();
         std::vector<const SupernodalFactor*> fp;
         std::vector<int> fo;
         for (int s = 0; s < H.nsub; ++s) if (H.Aii[l][s].rows > 0) { fp.push_back(&fi[s]); fo.push_back(off[s]); }

@@ -41,7 +41,7 @@ class PcgReport(C.Structure):
 EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
            "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
-           "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds"]
+           "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats"]
 
 
 def lib():
@@ -74,6 +74,8 @@ def lib():
         L.hdd_sigma_smooth.argtypes = [P, I, P, P]
         L.hdd_pcg_solve.argtypes = [P, C.POINTER(PcgParams), C.POINTER(PcgReport), P, I]
         L.hdd_setup_seconds.argtypes = [P, C.POINTER(C.c_double)]
+        L.hdd_profile_sweep.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hdd_stats.argtypes = [P, P, I]
         for name in EXPORTS:
             if name != "hdd_last_error":
                 getattr(L, name).restype = I
@@ -211,6 +213,18 @@ class DDSolver:
         self._check(lib().hdd_pcg_solve(self.h, C.byref(p), C.byref(rep), _ptr(hist), max_iterations + 1), ok=(0, 4))
         n = abs(rep.iterations) + 1
         return rep.iterations, hist[:n], rep.seconds
+
+    def profile_sweep(self, reps: int = 50):
+        """Mean duration (ms) of one level-0 smoothing sweep and its algorithmic bytes."""
+        ms, by = C.c_double(), C.c_double()
+        self._check(lib().hdd_profile_sweep(self.h, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def stats(self):
+        out = np.zeros(7, dtype=np.float64)
+        self._check(lib().hdd_stats(self.h, _ptr(out), 7))
+        keys = ["setup_s", "factor_s", "sigma_device_bytes", "pages", "interface_dofs", "interior_dofs", "subdomains"]
+        return dict(zip(keys, out.tolist()))
 
     # reference-facade spellings
     Solve = solve

@@ -140,6 +140,18 @@ hdd_status hdd_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* params, hdd_pcg_rep
 /* Setup wall time in seconds. */
 hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* seconds);
 
+/* Measurement: runs `reps` level-0 smoothing sweeps (all Active cells) on the
+ * context stream bracketed by HIP events; returns the mean duration of one
+ * launch and the algorithmic HBM bytes of one launch (16 B per page slot:
+ * flags, u and b read, u written). No reference counterpart. */
+hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes_per_launch);
+
+/* Setup statistics: out[0] setup s, out[1] factorisation s, out[2] device
+ * bytes of the interface solver, out[3] level-0 pages, out[4] interface dofs
+ * (level 0), out[5] interior dofs (level 0, all subdomains), out[6] subdomains.
+ * n = number of slots provided (up to 7). No reference counterpart. */
+hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,161 @@
+"""Pins the oracle (CPU restatement of the reference dd_pcg) against facts that
+do not come from its own code: independent cell counts, a numpy-assembled
+7-point operator, scipy direct solves, and algebraic invariants (symmetry,
+linearity, Schur complements at uniform stride)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from hierarchicaldd_amd import domain
+from tests.reference_ops import cell_coords, global_operator
+
+
+@pytest.fixture(scope="module")
+def box32(oracle_lib):
+    dom, cfg = domain.small_box(32, 16, 3, seed=1)
+    return dom, cfg, oracle_lib.Oracle(dom, cfg)
+
+
+def test_config0_counts_match_independent_survey():
+    """The 64^3 config: 1,003 pages, 221,251 Active cells (counted independently of this code)."""
+    dom, cfg = domain.config0()
+    assert dom.n_pages == 1003
+    assert int(((dom.flags & domain.ACTIVE) != 0).sum()) == 221251
+
+
+def test_sphere_active_count_closed_form():
+    n, r, c = 40, 9, (20, 21, 19)
+    dom = domain.sphere_obstacle_box(n, [(c[0], c[1], c[2], r)])
+    i = np.arange(1, n - 1)
+    X, Y, Z = np.meshgrid(i, i, i, indexing="ij")
+    inside = (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 <= r * r
+    assert int(((dom.flags & domain.ACTIVE) != 0).sum()) == (n - 2) ** 3 - int(inside.sum())
+
+
+def test_divider_marks_planes(box32):
+    dom, cfg, o = box32
+    off, fl = o.level(0)
+    x, y, z = cell_coords(off)
+    sub = cfg.sub_size
+    was_active = np.zeros_like(fl, dtype=bool)
+    inp = dict(zip(dom.page_offsets.tolist(), range(dom.n_pages)))
+    for p, po in enumerate(off.tolist()):
+        was_active[p] = (dom.flags[inp[po]] & domain.ACTIVE) != 0
+    on_plane = (x % sub == 0) | (y % sub == 0) | (z % sub == 0)
+    assert np.array_equal((fl & domain.INTERFACE) != 0, was_active & on_plane)
+    assert np.array_equal((fl & domain.ACTIVE) != 0, was_active & ~on_plane)
+    ns = (dom.size[0] - 1 + sub - 1) // sub
+    pid = ((x // sub) * ns + (y // sub)) * ns + (z // sub) + 1
+    act = (fl & domain.ACTIVE) != 0
+    assert np.array_equal((fl & 0xffff)[act], pid[act])
+
+
+def test_pcg_true_residual_and_scipy(box32):
+    """The oracle's PCG solution solves the numpy-assembled global system."""
+    dom, cfg, o = box32
+    tol = 1e-6 * float(np.abs(dom.b).max())
+    it, hist, _ = o.solve(tol, 200)
+    assert it > 0 and hist[-1] <= tol
+    off, fl = o.level(0)
+    A, unk = global_operator(off, fl)
+    b = o.get(4).reshape(-1)[unk]
+    x = o.x().reshape(-1)[unk]
+    true_res = np.abs(b - A @ x).max()
+    assert true_res <= 2 * tol
+    xs = spl.spsolve(A.tocsc(), b)
+    assert np.abs(x - xs).max() <= 1e-3 * np.abs(xs).max()
+
+
+def test_vcycle_is_linear_and_symmetric(box32):
+    dom, cfg, o = box32
+    off, fl = o.level(0)
+    rng = np.random.default_rng(0)
+    mask = (fl & (domain.ACTIVE | domain.INTERFACE)) != 0
+    u = np.where(mask, rng.standard_normal(fl.shape), 0).astype(np.float32)
+    v = np.where(mask, rng.standard_normal(fl.shape), 0).astype(np.float32)
+    Mu, Mv = o.vcycle(u), o.vcycle(v)
+    a, b = float(np.sum(Mu * v)), float(np.sum(u * Mv))
+    assert abs(a - b) <= 1e-4 * max(abs(a), abs(b))
+    assert np.allclose(o.vcycle(2 * u), 2 * Mu, rtol=0, atol=0)
+
+
+def test_preconditioner_symmetric_positive(box32):
+    """M^{-1} of the DD preconditioner is symmetric (symmetric V-cycles, exact solves) and positive."""
+    dom, cfg, o = box32
+    off, fl = o.level(0)
+    rng = np.random.default_rng(1)
+    mask = (fl & (domain.ACTIVE | domain.INTERFACE)) != 0
+    u = np.where(mask, rng.standard_normal(fl.shape), 0).astype(np.float32)
+    v = np.where(mask, rng.standard_normal(fl.shape), 0).astype(np.float32)
+    Mu, Mv = o.apply_preconditioner(u), o.apply_preconditioner(v)
+    a, b = float(np.sum(Mu * v)), float(np.sum(u * Mv))
+    assert abs(a - b) <= 2e-3 * max(abs(a), abs(b))
+    assert float(np.sum(Mu * u)) > 0
+
+
+def _csr(t):
+    r, c, p, i, v = t
+    return sp.csr_matrix((v, i, p), shape=(r, c))
+
+
+def test_uniform_stride_operators_are_fine_blocks(oracle_lib):
+    """sub_size 4 has stride 1 everywhere, so the interface operators are blocks of
+    the fine 7-point operator: Arr = A_GG and the Schur complement of the
+    subdomain blocks equals the global one, A_GG - A_GI A_II^-1 A_IG."""
+    dom, _ = domain.small_box(12, 4, 1, seed=3)
+    cfg = domain.DDConfig(sub_size=4, vlevels=1, sigma_levels=1)
+    o = oracle_lib.Oracle(dom, cfg)
+    off, fl = o.level(0)
+    A, unk = global_operator(off, fl)
+    x, y, z = (c.reshape(-1)[unk] for c in cell_coords(off))
+    gi = {(int(a), int(b), int(c)): k for k, (a, b, c) in enumerate(zip(x, y, z))}
+    ic = o.interface_coords(0)
+    G = np.array([gi[tuple(map(int, c))] for c in ic])
+    Arr = _csr(o.matrix("Arr", 0)).toarray()
+    Ad = A.toarray()
+    assert np.array_equal(Arr, Ad[np.ix_(G, G)])
+    I = np.setdiff1d(np.arange(len(unk)), G)
+    S_glob = Ad[np.ix_(G, G)] - Ad[np.ix_(G, I)] @ np.linalg.solve(Ad[np.ix_(I, I)], Ad[np.ix_(I, G)])
+    S = Arr.copy()
+    for s in range(o.n_subdomains):
+        Aii = _csr(o.matrix("Aii", 0, s)).toarray()
+        if Aii.shape[0] == 0:
+            continue
+        Air = _csr(o.matrix("Air", 0, s)).toarray()
+        S -= Air.T @ np.linalg.solve(Aii, Air)
+    assert np.abs(S - S_glob).max() <= 1e-9 * np.abs(S_glob).max()
+
+
+def test_interface_operator_invariants(box32):
+    dom, cfg, o = box32
+    for l in range(o.sigma_levels):
+        Arr = _csr(o.matrix("Arr", l)).toarray()
+        assert np.array_equal(Arr, Arr.T)
+        assert np.linalg.eigvalsh(Arr + 1e-5 * np.eye(len(Arr))).min() > 0
+        for s in range(o.n_subdomains):
+            Aii = _csr(o.matrix("Aii", l, s)).toarray()
+            assert np.array_equal(Aii, Aii.T)
+        if l + 1 < o.sigma_levels:
+            P = _csr(o.matrix("Prr", l)).toarray()
+            assert np.all(P.sum(axis=1) <= 1 + 1e-7) and np.all(P >= 0)
+
+
+def test_sigma_smooth_is_the_shifted_schur_iteration(box32):
+    """Smooth: u <- (Arr+eI)^-1 (b + sum_s Air^T (Aii+eI)^-1 Air u), e = 1e-5 (IFS/SIGMA_SMOOTHER.cpp)."""
+    dom, cfg, o = box32
+    l = o.sigma_levels - 1
+    n = o.interface_dofs(l)
+    Arr = _csr(o.matrix("Arr", l)).toarray()
+    K = np.zeros((n, n))
+    for s in range(o.n_subdomains):
+        Aii = _csr(o.matrix("Aii", l, s)).toarray()
+        if Aii.shape[0] == 0:
+            continue
+        Air = _csr(o.matrix("Air", l, s)).toarray()
+        K += Air.T @ np.linalg.solve(Aii + 1e-5 * np.eye(len(Aii)), Air)
+    rng = np.random.default_rng(2)
+    b, u = rng.standard_normal(n), rng.standard_normal(n)
+    expect = np.linalg.solve(Arr + 1e-5 * np.eye(n), b + K @ u)
+    got = o.sigma_smooth(l, b, u)
+    assert np.abs(got - expect).max() <= 1e-4 * np.abs(expect).max()
