@@ -57,7 +57,7 @@ def main():
     solver = hdd.DDSolver(dom, cfg)
     st = solver.stats()
     log(f"[bench] setup {time.time() - t0:.1f}s (factorisation {st['factor_s']:.1f}s, interface-solver plans "
-        f"{st['sigma_device_bytes'] / 2**30:.2f} GiB)")
+        f"{st['sigma_device_bytes'] / 2**30:.2f} GiB, bottom rho {st['bottom_rho']:.4f}, direct bottom {bool(st['bottom_direct'])})")
     tol = 1e-6 * float(np.abs(dom.b).max())
 
     for w in range(args.warmup):

@@ -233,10 +233,11 @@ hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* byt
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
     if (!ctx || !out) return HDD_ERR_ARG;
     const Solver& S = *ctx->solver;
-    double v[7] = {S.setup_seconds, S.sigma ? S.sigma->factor_seconds : 0.0, S.sigma ? (double)S.sigma->plan_bytes : 0.0,
+    double v[9] = {S.setup_seconds, S.sigma ? S.sigma->factor_seconds : 0.0, S.sigma ? (double)S.sigma->plan_bytes : 0.0,
                    (double)S.lv[0].host.npages(), S.sigma_host.levels ? (double)S.sigma_host.n_if[0] : 0.0,
-                   (S.sigma && S.sigma->levels) ? (double)S.sigma->lv[0].n_int : 0.0, (double)S.sigma_host.nsub};
-    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+                   (S.sigma && S.sigma->levels) ? (double)S.sigma->lv[0].n_int : 0.0, (double)S.sigma_host.nsub,
+                   S.sigma ? S.sigma->bottom_rho : -1.0, S.sigma ? (double)S.sigma->bottom_direct : 0.0};
+    for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
     return HDD_OK;
 }
 

@@ -22,6 +22,7 @@ struct CsrDev {
 struct TriPlan {
     int n = 0, H = -1;
     std::vector<int> lev_begin;                 // rows of height h: rowlist[lev_begin[h] .. lev_begin[h+1])
+    std::vector<int> lanes;                     // lanes per row, [height][fwd gather, fwd diag, bwd gather, bwd diag]
     DBuf<int> orig_of, sn_of, s0, sz, nr, roff, srows, rowlist, lp, li;
     DBuf<long long> doff, loff;
     DBuf<float> diag, diagT, l21, lv, y, t;
@@ -39,6 +40,15 @@ struct SigmaLevel {
 
 struct SigmaDevice {
     int levels = 0, smoothing = 3, n_mg = 1;
+    // Bottom level: the reference runs 200 smooths from u = 0, a fixed-point
+    // iteration with contraction rho = spectral radius of (Arr+eI)^-1 K. When
+    // rho^200 is far below float precision the result equals, to rounding, the
+    // solution of the bottom Schur system, computed here by one solve of the
+    // combined system [[Aii+eI, Air], [Air^T, Arr+eI]].
+    bool bottom_direct = false;
+    double bottom_rho = -1;
+    TriPlan Lbottom;
+    DBuf<float> bot_rhs, bot_sol;
     std::vector<SigmaLevel> lv;
     float* b0 = nullptr;   // level-0 interface rhs (lv[0].b)
     float* u0 = nullptr;   // level-0 interface solution (lv[0].u)

@@ -21,6 +21,9 @@
 #include <algorithm>
 #include <memory>
 #include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <string>
 #include <omp.h>
 #include "hdd_factor.h"
 #include "hdd_parallel.h"
@@ -31,57 +34,72 @@ namespace hdd {
 
 namespace {
 
+// Every kernel below gives one row to a group of G lanes (G = 4..64, a power of
+// two chosen per height level from the level's mean row length), so a 64-lane
+// wavefront works on 64/G rows at once; the group reduces with shuffles of
+// width G.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_down(v, o, G);
+    return v;
+}
+
+template <int G>
 __global__ void __launch_bounds__(256) k_tri_fwd_gather(const int* __restrict__ rows, int nrows, const int* __restrict__ orig_of,
                                                         const float* __restrict__ b, const int* __restrict__ lp, const int* __restrict__ li,
                                                         const float* __restrict__ lv, const float* __restrict__ y, float* __restrict__ t) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w];
     float s = 0.f;
-    for (int e = lp[i] + lane; e < lp[i + 1]; e += 64) s += lv[e] * y[li[e]];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    for (int e = lp[i] + lane; e < lp[i + 1]; e += G) s += lv[e] * y[li[e]];
+    s = group_sum<G>(s);
     if (lane == 0) t[i] = b[orig_of[i]] - s;
 }
 
+template <int G>
 __global__ void __launch_bounds__(256) k_tri_fwd_diag(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
                                                       const int* __restrict__ s0, const int* __restrict__ sz, const long long* __restrict__ doff,
                                                       const float* __restrict__ diag, const float* __restrict__ t, float* __restrict__ y) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w], s = sn_of[i], base = s0[s], n = sz[s], r = i - base;
     const float* D = diag + doff[s] + (long long)r * n;
     float acc = 0.f;
-    for (int k = lane; k <= r; k += 64) acc += D[k] * t[base + k];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    for (int k = lane; k <= r; k += G) acc += D[k] * t[base + k];
+    acc = group_sum<G>(acc);
     if (lane == 0) y[i] = acc;
 }
 
+template <int G>
 __global__ void __launch_bounds__(256) k_tri_bwd_gather(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
                                                         const int* __restrict__ s0, const int* __restrict__ nr, const long long* __restrict__ loff,
                                                         const int* __restrict__ roff, const int* __restrict__ srows, const float* __restrict__ l21,
                                                         const float* __restrict__ y, float* __restrict__ t) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w], s = sn_of[i], r = i - s0[s], m = nr[s];
     const float* Lc = l21 + loff[s] + (long long)r * m;
     const int* R = srows + roff[s];
     float acc = 0.f;
-    for (int a = lane; a < m; a += 64) acc += Lc[a] * y[R[a]];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    for (int a = lane; a < m; a += G) acc += Lc[a] * y[R[a]];
+    acc = group_sum<G>(acc);
     if (lane == 0) t[i] = y[i] - acc;
 }
 
+template <int G>
 __global__ void __launch_bounds__(256) k_tri_bwd_diag(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
                                                       const int* __restrict__ s0, const int* __restrict__ sz, const long long* __restrict__ doff,
                                                       const float* __restrict__ diagT, const float* __restrict__ t, float* __restrict__ y,
                                                       const int* __restrict__ orig_of, float* __restrict__ out) {
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w], s = sn_of[i], base = s0[s], n = sz[s], r = i - base;
     const float* D = diagT + doff[s] + (long long)r * n;
     float acc = 0.f;
-    for (int k = r + lane; k < n; k += 64) acc += D[k] * t[base + k];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    for (int k = r + lane; k < n; k += G) acc += D[k] * t[base + k];
+    acc = group_sum<G>(acc);
     if (lane == 0) { y[i] = acc; out[orig_of[i]] = acc; }
 }
 
@@ -213,6 +231,20 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
             if (h_height[g] == h) for (int k = 0; k < h_sz[g]; ++k) h_rows.push_back(h_s0[g] + k);
     }
     lev_begin[H + 1] = (int)h_rows.size();
+    // lanes per row for each level and phase, from the mean work per row
+    auto lanes_for = [](double len) { int g = 4; while (g < 64 && g * 3 < len) g *= 2; return g; };
+    lanes.assign((size_t)(H + 1) * 4, 4);
+    for (int h = 0; h <= H; ++h) {
+        double fg = 0, fd = 0, bg = 0, bd = 0;
+        const int cnt = lev_begin[h + 1] - lev_begin[h];
+        if (!cnt) continue;
+        for (int q = lev_begin[h]; q < lev_begin[h + 1]; ++q) {
+            const int i = h_rows[q], g = h_snof[i], r = i - h_s0[g];
+            fg += lcount[i + 1] - lcount[i]; fd += r + 1; bg += h_nr[g]; bd += h_sz[g] - r;
+        }
+        lanes[h * 4 + 0] = lanes_for(fg / cnt); lanes[h * 4 + 1] = lanes_for(fd / cnt);
+        lanes[h * 4 + 2] = lanes_for(bg / cnt); lanes[h * 4 + 3] = lanes_for(bd / cnt);
+    }
     orig_of.upload(h_orig); sn_of.upload(h_snof); s0.upload(h_s0); sz.upload(h_sz); nr.upload(h_nr);
     roff.upload(h_roff); srows.upload(h_srows); rowlist.upload(h_rows); lp.upload(lcount); li.upload(h_li);
     doff.upload(h_doff); loff.upload(h_loff);
@@ -225,25 +257,102 @@ size_t TriPlan::bytes() const {
            (diag.n + diagT.n + l21.n + lv.n + y.n + t.n) * 4;
 }
 
+#define HDD_LANES_DISPATCH(G, KERNEL, GRIDROWS, ...)                                                     \
+    switch (G) {                                                                                        \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(blocks_for(GRIDROWS * 4, 256)), dim3(256), 0, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, dim3(blocks_for(GRIDROWS * 8, 256)), dim3(256), 0, __VA_ARGS__); break;   \
+        case 16: hipLaunchKernelGGL(KERNEL<16>, dim3(blocks_for(GRIDROWS * 16, 256)), dim3(256), 0, __VA_ARGS__); break; \
+        case 32: hipLaunchKernelGGL(KERNEL<32>, dim3(blocks_for(GRIDROWS * 32, 256)), dim3(256), 0, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(KERNEL<64>, dim3(blocks_for(GRIDROWS * 64, 256)), dim3(256), 0, __VA_ARGS__); break; \
+    }
+
 void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     if (n == 0) return;
     for (int h = 0; h <= H; ++h) {
         const int cnt = lev_begin[h + 1] - lev_begin[h];
         if (!cnt) continue;
         const int* rows = rowlist.p + lev_begin[h];
-        hipLaunchKernelGGL(k_tri_fwd_gather, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, orig_of.p, b, lp.p, li.p, lv.p, y.p, t.p);
-        hipLaunchKernelGGL(k_tri_fwd_diag, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diag.p, t.p, y.p);
+        HDD_LANES_DISPATCH(lanes[h * 4 + 0], k_tri_fwd_gather, cnt, s, rows, cnt, orig_of.p, b, lp.p, li.p, lv.p, y.p, t.p);
+        HDD_LANES_DISPATCH(lanes[h * 4 + 1], k_tri_fwd_diag, cnt, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diag.p, t.p, y.p);
     }
     for (int h = H; h >= 0; --h) {
         const int cnt = lev_begin[h + 1] - lev_begin[h];
         if (!cnt) continue;
         const int* rows = rowlist.p + lev_begin[h];
-        hipLaunchKernelGGL(k_tri_bwd_gather, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, sn_of.p, s0.p, nr.p, loff.p, roff.p, srows.p, l21.p, y.p, t.p);
-        hipLaunchKernelGGL(k_tri_bwd_diag, dim3(blocks_for(cnt, 4)), dim3(256), 0, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diagT.p, t.p, y.p, orig_of.p, x);
+        HDD_LANES_DISPATCH(lanes[h * 4 + 2], k_tri_bwd_gather, cnt, s, rows, cnt, sn_of.p, s0.p, nr.p, loff.p, roff.p, srows.p, l21.p, y.p, t.p);
+        HDD_LANES_DISPATCH(lanes[h * 4 + 3], k_tri_bwd_diag, cnt, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diagT.p, t.p, y.p, orig_of.p, x);
     }
 }
 
 SigmaDevice::~SigmaDevice() {}
+
+// spectral radius of (Arr+eI)^-1 sum_s Air_s^T (Aii_s+eI)^-1 Air_s by power
+// iteration with the double-precision factors (upper estimate: last growth ratio)
+static double estimate_bottom_rho(const SigmaHost& H, int l, const std::vector<SupernodalFactor>& fi, const SupernodalFactor& fr) {
+    const int n = H.n_if[l];
+    if (n == 0) return 0.0;
+    std::vector<double> x(n), y(n);
+    uint64_t st = 0x9e3779b97f4a7c15ULL;
+    for (int i = 0; i < n; ++i) { st = st * 6364136223846793005ULL + 1442695040888963407ULL; x[i] = (double)((st >> 11) & 0xfffff) / 1048576.0 - 0.5; }
+    auto normalize = [&](std::vector<double>& v) { double s = 0; for (double a : v) s += a * a; s = std::sqrt(s); for (double& a : v) a /= s; return s; };
+    normalize(x);
+    double rho = 0, prev = -1;
+    for (int it = 0; it < 60; ++it) {
+        std::vector<std::vector<double>> contrib(H.nsub);
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < H.nsub; ++s) {
+            const Csr& A = H.Air[l][s];
+            if (H.Aii[l][s].rows == 0) continue;
+            std::vector<double> vb(A.rows), vx(A.rows);
+            for (int i = 0; i < A.rows; ++i) { double a = 0; for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) a += (double)A.val[e] * x[A.idx[e]]; vb[i] = a; }
+            factor_solve(fi[s], vb.data(), vx.data());
+            contrib[s].swap(vx);
+        }
+        std::vector<double> k(n, 0.0);
+        for (int s = 0; s < H.nsub; ++s) {
+            const Csr& A = H.Air[l][s];
+            if (contrib[s].empty()) continue;
+            for (int i = 0; i < A.rows; ++i) for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) k[A.idx[e]] += (double)A.val[e] * contrib[s][i];
+        }
+        factor_solve(fr, k.data(), y.data());
+        rho = normalize(y);
+        x.swap(y);
+        if (it > 8 && std::fabs(rho - prev) < 1e-6 * rho) break;
+        prev = rho;
+    }
+    return rho;
+}
+
+// combined bottom system [[Aii (all subdomains), Air], [Air^T, Arr]] and its coordinates
+static void combined_bottom(const SigmaHost& H, int l, Csr& C, std::vector<Int3>& xyz) {
+    std::vector<int> off(H.nsub + 1, 0);
+    for (int s = 0; s < H.nsub; ++s) off[s + 1] = off[s] + H.Aii[l][s].rows;
+    const int ni = off[H.nsub], nf = H.n_if[l], n = ni + nf;
+    std::vector<std::vector<std::pair<int, float>>> rows(n);
+    for (int s = 0; s < H.nsub; ++s) {
+        const Csr& Ai = H.Aii[l][s];
+        const Csr& Ar = H.Air[l][s];
+        for (int i = 0; i < Ai.rows; ++i) {
+            for (int e = Ai.ptr[i]; e < Ai.ptr[i + 1]; ++e) rows[off[s] + i].push_back({off[s] + Ai.idx[e], Ai.val[e]});
+            for (int e = Ar.ptr[i]; e < Ar.ptr[i + 1]; ++e) {
+                rows[off[s] + i].push_back({ni + Ar.idx[e], Ar.val[e]});
+                rows[ni + Ar.idx[e]].push_back({off[s] + i, Ar.val[e]});
+            }
+        }
+    }
+    const Csr& R = H.Arr[l];
+    for (int i = 0; i < nf; ++i)
+        for (int e = R.ptr[i]; e < R.ptr[i + 1]; ++e) rows[ni + i].push_back({ni + R.idx[e], R.val[e]});
+    C = Csr(); C.rows = C.cols = n; C.ptr.assign(1, 0);
+    for (auto& r : rows) {
+        std::sort(r.begin(), r.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (auto& p : r) { C.idx.push_back(p.first); C.val.push_back(p.second); }
+        C.ptr.push_back((int)C.idx.size());
+    }
+    xyz.clear();
+    for (int s = 0; s < H.nsub; ++s) xyz.insert(xyz.end(), H.Aii_coords[l][s].begin(), H.Aii_coords[l][s].end());
+    for (int i = 0; i < nf; ++i) xyz.push_back({H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l});
+}
 
 std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing, int n_mg) {
     std::unique_ptr<SigmaDevice> S(new SigmaDevice());
@@ -279,12 +388,30 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing
         std::vector<int> fo;
         for (int s = 0; s < H.nsub; ++s) if (H.Aii[l][s].rows > 0) { fp.push_back(&fi[s]); fo.push_back(off[s]); }
         L.Lii.build(fp, fo);
-        std::vector<SupernodalFactor>().swap(fi);
         std::vector<Int3> rc(H.if_coords[l].size());
         for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l};
         SupernodalFactor fr;
         factor_spd(H.Arr[l], rc, 1e-5, fr);
         L.Lrr.build({&fr}, {0});
+        if (l == H.levels - 1 && L.n_if > 0) {
+            const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
+            const std::string m = mode ? mode : "auto";
+            if (m != "loop") {
+                S->bottom_rho = estimate_bottom_rho(H, l, fi, fr);
+                // 2% margin on the estimate; 1e-12 is far below float resolution
+                S->bottom_direct = (m == "direct") || std::pow(std::min(1.0, 1.02 * S->bottom_rho), 200.0) < 1e-12;
+            }
+            if (S->bottom_direct) {
+                Csr C; std::vector<Int3> xyz;
+                combined_bottom(H, l, C, xyz);
+                SupernodalFactor fc;
+                factor_spd(C, xyz, 1e-5, fc);
+                S->Lbottom.build({&fc}, {0});
+                S->bot_rhs.alloc(C.rows); S->bot_sol.alloc(C.rows);
+                S->plan_bytes += S->Lbottom.bytes();
+            }
+        }
+        std::vector<SupernodalFactor>().swap(fi);
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
         S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
@@ -330,7 +457,16 @@ void SigmaDevice::vcycle(hipStream_t s) {
             hipLaunchKernelGGL(k_csr_mv, dim3(blocks_for(C.n_if, 256)), dim3(256), 0, s, C.n_if, lv[l].PT.p.p, lv[l].PT.ix.p, lv[l].PT.v.p,
                                (const float*)lv[l].r.p, C.b.p);
     }
-    for (int i = 0; i < 200; ++i) smooth(NL - 1, s);
+    if (bottom_direct) {
+        // u = bottom part of [[Aii+eI, Air],[Air^T, Arr+eI]]^{-1} [0; b]; the interior part of bot_rhs stays zero
+        SigmaLevel& B = lv[NL - 1];
+        const int ni = B.n_int;
+        launch_copy(bot_rhs.p + ni, B.b.p, B.n_if, s);
+        Lbottom.solve(bot_rhs.p, bot_sol.p, s);
+        launch_copy(B.u.p, bot_sol.p + ni, B.n_if, s);
+    } else {
+        for (int i = 0; i < 200; ++i) smooth(NL - 1, s);
+    }
     for (int l = NL - 2; l >= 0; --l) {
         SigmaLevel& F = lv[l];
         if (F.n_if)

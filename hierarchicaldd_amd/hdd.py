@@ -221,9 +221,10 @@ class DDSolver:
         return ms.value, by.value
 
     def stats(self):
-        out = np.zeros(7, dtype=np.float64)
-        self._check(lib().hdd_stats(self.h, _ptr(out), 7))
-        keys = ["setup_s", "factor_s", "sigma_device_bytes", "pages", "interface_dofs", "interior_dofs", "subdomains"]
+        out = np.zeros(9, dtype=np.float64)
+        self._check(lib().hdd_stats(self.h, _ptr(out), 9))
+        keys = ["setup_s", "factor_s", "sigma_device_bytes", "pages", "interface_dofs", "interior_dofs", "subdomains",
+                "bottom_rho", "bottom_direct"]
         return dict(zip(keys, out.tolist()))
 
     # reference-facade spellings

@@ -148,8 +148,11 @@ hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* byt
 
 /* Setup statistics: out[0] setup s, out[1] factorisation s, out[2] device
  * bytes of the interface solver, out[3] level-0 pages, out[4] interface dofs
- * (level 0), out[5] interior dofs (level 0, all subdomains), out[6] subdomains.
- * n = number of slots provided (up to 7). No reference counterpart. */
+ * (level 0), out[5] interior dofs (level 0, all subdomains), out[6] subdomains,
+ * out[7] estimated contraction of the bottom sigma smoother (-1 if not
+ * estimated), out[8] 1 if the bottom level is solved directly (its 200 smooths
+ * have converged below float resolution), else 0.
+ * n = number of slots provided (up to 9). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
 #ifdef __cplusplus
