@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--config", default="sphere256", choices=["sphere256", "sphere64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=500)
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="set up, then run only level-0 sweeps and a plain copy of known size (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -59,6 +61,13 @@ def main():
     log(f"[bench] setup {time.time() - t0:.1f}s (factorisation {st['factor_s']:.1f}s, interface-solver plans "
         f"{st['sigma_device_bytes'] / 2**30:.2f} GiB, bottom rho {st['bottom_rho']:.4f}, direct bottom {bool(st['bottom_direct'])})")
     tol = 1e-6 * float(np.abs(dom.b).max())
+
+    if args.sweep_only:
+        sweep_ms, sweep_bytes = solver.profile_sweep(20)
+        log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic")
+        copy_ms, copy_bytes = solver.profile_copy(20)
+        log(f"[bench] calibration copy: {copy_ms * 1e3:.1f} us/launch, {copy_bytes / 2:.0f} bytes read and {copy_bytes / 2:.0f} written")
+        return
 
     for w in range(args.warmup):
         it, hist, secs = solver.solve(tol, args.max_iterations)

@@ -2,6 +2,7 @@
 #include "../../include/hdd.h"
 #include "hdd_runtime.h"
 #include "hdd_sigma_device.h"
+#include "hdd_spectral.h"
 #include <cstring>
 #include <string>
 
@@ -230,15 +231,61 @@ hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* byt
     })
 }
 
+hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx || reps <= 0 || !avg_ms) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        Solver& S = *ctx->solver;
+        LevelBuffers& L = S.lv[0];
+        const size_t n = L.cells();
+        // the smoother's spare ping-pong buffer is overwritten by the next sweep before it is read
+        float* dst = L.u[1 - L.cur].p;
+        const float* src = L.u[L.cur].p;
+        hipEvent_t a, b;
+        hdd_check(hipEventCreate(&a)); hdd_check(hipEventCreate(&b));
+        launch_copy(dst, src, n, S.stream);
+        hdd_check(hipEventRecord(a, S.stream));
+        for (int i = 0; i < reps; ++i) launch_copy(dst, src, n, S.stream);
+        hdd_check(hipEventRecord(b, S.stream));
+        hdd_check(hipEventSynchronize(b));
+        float ms = 0.f;
+        hdd_check(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+        *avg_ms = ms / reps;
+        if (bytes) *bytes = 8.0 * n;
+    })
+}
+
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
     if (!ctx || !out) return HDD_ERR_ARG;
     const Solver& S = *ctx->solver;
-    double v[9] = {S.setup_seconds, S.sigma ? S.sigma->factor_seconds : 0.0, S.sigma ? (double)S.sigma->plan_bytes : 0.0,
-                   (double)S.lv[0].host.npages(), S.sigma_host.levels ? (double)S.sigma_host.n_if[0] : 0.0,
-                   (S.sigma && S.sigma->levels) ? (double)S.sigma->lv[0].n_int : 0.0, (double)S.sigma_host.nsub,
-                   S.sigma ? S.sigma->bottom_rho : -1.0, S.sigma ? (double)S.sigma->bottom_direct : 0.0};
-    for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
+    const SigmaDevice* G = S.sigma.get();
+    double v[13] = {S.setup_seconds, G ? G->factor_seconds : 0.0, G ? (double)G->plan_bytes : 0.0,
+                    (double)S.lv[0].host.npages(), S.sigma_host.levels ? (double)S.sigma_host.n_if[0] : 0.0,
+                    (G && G->levels) ? (double)G->lv[0].n_int : 0.0, (double)S.sigma_host.nsub,
+                    G ? G->bottom_rho : -1.0, G ? (double)G->bottom_direct : 0.0, G ? (double)G->bottom_k : 0.0,
+                    G ? G->bottom_seconds : 0.0, G ? (double)G->bottom_steps : 0.0, G ? G->bottom_check : -1.0};
+    for (int i = 0; i < n && i < 13; ++i) out[i] = v[i];
     return HDD_OK;
+}
+
+hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max) {
+    if (!ctx || !k || cap < 0 || (cap > 0 && !lambdas) || !(lambda_cut > 0.0 && lambda_cut < 1.0)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        const SigmaHost& H = ctx->solver->sigma_host;
+        if (H.levels == 0) throw std::runtime_error("no sigma levels");
+        const int l = H.levels - 1;
+        Csr C; std::vector<Int3> xyz;
+        combined_bottom(H, l, C, xyz);
+        SupernodalFactor fc;
+        factor_spd(C, xyz, 1e-5, fc);
+        BottomSpectrum sp;
+        bottom_spectrum(H.Arr[l], 1e-5, fc, C.rows - H.n_if[l], lambda_cut, size_t(2) << 30, sp);
+        if (!sp.resolved) throw std::runtime_error("bottom spectrum not resolved (too many slow modes or basis exhausted)");
+        *k = (int)sp.lambda.size();
+        for (int j = 0; j < *k && j < cap; ++j) lambdas[j] = sp.lambda[j];
+        if (lambda_max) *lambda_max = sp.lambda_max;
+    })
 }
 
 }  // extern "C"

@@ -40,15 +40,23 @@ struct SigmaLevel {
 
 struct SigmaDevice {
     int levels = 0, smoothing = 3, n_mg = 1;
-    // Bottom level: the reference runs 200 smooths from u = 0, a fixed-point
-    // iteration with contraction rho = spectral radius of (Arr+eI)^-1 K. When
-    // rho^200 is far below float precision the result equals, to rounding, the
-    // solution of the bottom Schur system, computed here by one solve of the
-    // combined system [[Aii+eI, Air], [Air^T, Arr+eI]].
+    // Bottom level: the reference runs 200 smooths, a stationary iteration
+    // u <- (Arr+eI)^-1 (b + K u) whose iteration matrix M has eigenvalues in
+    // [0, 1). Its result is x* + M^200 (u0 - x*) with x* the solution of the
+    // bottom Schur system, computed here by one solve of the combined system
+    // [[Aii+eI, Air], [Air^T, Arr+eI]], and M^200 kept on the k eigenmodes whose
+    // 200th power is above 1e-9 (V, S V and their weights, found at setup).
     bool bottom_direct = false;
-    double bottom_rho = -1;
+    double bottom_rho = -1;        // largest eigenvalue of M (-1: not computed)
+    int bottom_k = 0;              // eigenmodes kept
+    int bottom_steps = 0;          // block Krylov steps of the eigen-solve
+    double bottom_seconds = 0;     // host time of the eigen-solve
+    double bottom_check = -1;      // max |direct+correction - 200 sweeps| / max |200 sweeps| on a random rhs
     TriPlan Lbottom;
-    DBuf<float> bot_rhs, bot_sol;
+    DBuf<float> bot_rhs, bot_sol, spec_V, spec_SV, spec_coef;
+    DBuf<double> spec_w1, spec_w2, spec_part;
+    int spec_nch = 0;
+    void bottom(hipStream_t s, bool u_zero);
     std::vector<SigmaLevel> lv;
     float* b0 = nullptr;   // level-0 interface rhs (lv[0].b)
     float* u0 = nullptr;   // level-0 interface solution (lv[0].u)
@@ -57,7 +65,7 @@ struct SigmaDevice {
     ~SigmaDevice();
     void smooth(int l, hipStream_t s);
     void residual(int l, hipStream_t s);
-    void vcycle(hipStream_t s);
+    void vcycle(hipStream_t s, bool first);
 };
 
 std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& host, int smoothing, int n_mg);

@@ -29,6 +29,7 @@
 #include "hdd_parallel.h"
 #include "hdd_runtime.h"
 #include "hdd_sigma_device.h"
+#include "hdd_spectral.h"
 
 namespace hdd {
 
@@ -134,6 +135,51 @@ __global__ void k_prolong_add(int rows, const int* __restrict__ p, const int* __
     float acc = 0.f;
     for (int e = p[i]; e < p[i + 1]; ++e) acc += v[e] * uc[ix[e]];
     u[i] += acc;
+}
+
+// Low-rank part of the bottom solve: part[j][c] = (V_j . b, (S V)_j . u) over
+// rows [c*kSpecChunk, (c+1)*kSpecChunk), in double; u == nullptr means u = 0.
+constexpr int kSpecChunk = 4096;
+__global__ void __launch_bounds__(256) k_spec_dot(int n, int nch, const float* __restrict__ V, const float* __restrict__ SV,
+                                                  const float* __restrict__ b, const float* __restrict__ u, double* __restrict__ part) {
+    const int c = blockIdx.x, j = blockIdx.y;
+    const int r0 = c * kSpecChunk, r1 = min(n, r0 + kSpecChunk);
+    const float* vj = V + (size_t)j * n;
+    double sb = 0.0, su = 0.0;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) sb += (double)vj[i] * (double)b[i];
+    if (u) {
+        const float* sj = SV + (size_t)j * n;
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) su += (double)sj[i] * (double)u[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) { sb += __shfl_down(sb, o); su += __shfl_down(su, o); }
+    __shared__ double red[2][4];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = sb; red[1][wv] = su; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[((size_t)j * nch + c) * 2] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        part[((size_t)j * nch + c) * 2 + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
+}
+
+// coef_j = l_j^200 (v_j^T S u - v_j^T b / (1 - l_j)) = w2_j * su_j - w1_j * sb_j
+__global__ void k_spec_coef(int k, int nch, const double* __restrict__ part, const double* __restrict__ w1, const double* __restrict__ w2,
+                            float* __restrict__ coef) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    double sb = 0.0, su = 0.0;
+    for (int c = 0; c < nch; ++c) { sb += part[((size_t)j * nch + c) * 2]; su += part[((size_t)j * nch + c) * 2 + 1]; }
+    coef[j] = (float)(w2[j] * su - w1[j] * sb);
+}
+
+// u = x* + V coef
+__global__ void k_spec_apply(int n, int k, const float* __restrict__ V, const float* __restrict__ coef, const float* __restrict__ xs,
+                             float* __restrict__ u) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = xs[i];
+    for (int j = 0; j < k; ++j) acc += V[(size_t)j * n + i] * coef[j];
+    u[i] = acc;
 }
 
 }  // namespace
@@ -286,72 +332,47 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
 
 SigmaDevice::~SigmaDevice() {}
 
-// spectral radius of (Arr+eI)^-1 sum_s Air_s^T (Aii_s+eI)^-1 Air_s by power
-// iteration with the double-precision factors (upper estimate: last growth ratio)
-static double estimate_bottom_rho(const SigmaHost& H, int l, const std::vector<SupernodalFactor>& fi, const SupernodalFactor& fr) {
-    const int n = H.n_if[l];
-    if (n == 0) return 0.0;
-    std::vector<double> x(n), y(n);
-    uint64_t st = 0x9e3779b97f4a7c15ULL;
-    for (int i = 0; i < n; ++i) { st = st * 6364136223846793005ULL + 1442695040888963407ULL; x[i] = (double)((st >> 11) & 0xfffff) / 1048576.0 - 0.5; }
-    auto normalize = [&](std::vector<double>& v) { double s = 0; for (double a : v) s += a * a; s = std::sqrt(s); for (double& a : v) a /= s; return s; };
-    normalize(x);
-    double rho = 0, prev = -1;
-    for (int it = 0; it < 60; ++it) {
-        std::vector<std::vector<double>> contrib(H.nsub);
-#pragma omp parallel for schedule(dynamic)
-        for (int s = 0; s < H.nsub; ++s) {
-            const Csr& A = H.Air[l][s];
-            if (H.Aii[l][s].rows == 0) continue;
-            std::vector<double> vb(A.rows), vx(A.rows);
-            for (int i = 0; i < A.rows; ++i) { double a = 0; for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) a += (double)A.val[e] * x[A.idx[e]]; vb[i] = a; }
-            factor_solve(fi[s], vb.data(), vx.data());
-            contrib[s].swap(vx);
-        }
-        std::vector<double> k(n, 0.0);
-        for (int s = 0; s < H.nsub; ++s) {
-            const Csr& A = H.Air[l][s];
-            if (contrib[s].empty()) continue;
-            for (int i = 0; i < A.rows; ++i) for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) k[A.idx[e]] += (double)A.val[e] * contrib[s][i];
-        }
-        factor_solve(fr, k.data(), y.data());
-        rho = normalize(y);
-        x.swap(y);
-        if (it > 8 && std::fabs(rho - prev) < 1e-6 * rho) break;
-        prev = rho;
-    }
-    return rho;
+// 200 sweeps leave l^200 of an eigenmode with eigenvalue l; modes with l^200 < 1e-9 are dropped
+static const double kBottomLambdaCut = std::pow(1e-9, 1.0 / 200.0);
+static const size_t kBottomBasisBytes = size_t(2) << 30;
+
+static void upload_spectrum(SigmaDevice& S, const BottomSpectrum& sp, int n, bool keep_sv) {
+    const int k = (int)sp.lambda.size();
+    S.bottom_k = k;
+    if (k == 0) return;
+    std::vector<float> v(sp.V.begin(), sp.V.end());
+    S.spec_V.upload(v);
+    if (keep_sv) { std::vector<float> sv(sp.SV.begin(), sp.SV.end()); S.spec_SV.upload(sv); }
+    std::vector<double> w1(k), w2(k);
+    for (int j = 0; j < k; ++j) { w2[j] = std::pow(sp.lambda[j], 200.0); w1[j] = w2[j] / (1.0 - sp.lambda[j]); }
+    S.spec_w1.upload(w1); S.spec_w2.upload(w2);
+    S.spec_nch = (n + kSpecChunk - 1) / kSpecChunk;
+    S.spec_part.alloc((size_t)2 * k * S.spec_nch);
+    S.spec_coef.alloc(k);
+    S.plan_bytes += S.spec_V.n * 4 + S.spec_SV.n * 4;
 }
 
-// combined bottom system [[Aii (all subdomains), Air], [Air^T, Arr]] and its coordinates
-static void combined_bottom(const SigmaHost& H, int l, Csr& C, std::vector<Int3>& xyz) {
-    std::vector<int> off(H.nsub + 1, 0);
-    for (int s = 0; s < H.nsub; ++s) off[s + 1] = off[s] + H.Aii[l][s].rows;
-    const int ni = off[H.nsub], nf = H.n_if[l], n = ni + nf;
-    std::vector<std::vector<std::pair<int, float>>> rows(n);
-    for (int s = 0; s < H.nsub; ++s) {
-        const Csr& Ai = H.Aii[l][s];
-        const Csr& Ar = H.Air[l][s];
-        for (int i = 0; i < Ai.rows; ++i) {
-            for (int e = Ai.ptr[i]; e < Ai.ptr[i + 1]; ++e) rows[off[s] + i].push_back({off[s] + Ai.idx[e], Ai.val[e]});
-            for (int e = Ar.ptr[i]; e < Ar.ptr[i + 1]; ++e) {
-                rows[off[s] + i].push_back({ni + Ar.idx[e], Ar.val[e]});
-                rows[ni + Ar.idx[e]].push_back({off[s] + i, Ar.val[e]});
-            }
-        }
-    }
-    const Csr& R = H.Arr[l];
-    for (int i = 0; i < nf; ++i)
-        for (int e = R.ptr[i]; e < R.ptr[i + 1]; ++e) rows[ni + i].push_back({ni + R.idx[e], R.val[e]});
-    C = Csr(); C.rows = C.cols = n; C.ptr.assign(1, 0);
-    for (auto& r : rows) {
-        std::sort(r.begin(), r.end(), [](auto& a, auto& b) { return a.first < b.first; });
-        for (auto& p : r) { C.idx.push_back(p.first); C.val.push_back(p.second); }
-        C.ptr.push_back((int)C.idx.size());
-    }
-    xyz.clear();
-    for (int s = 0; s < H.nsub; ++s) xyz.insert(xyz.end(), H.Aii_coords[l][s].begin(), H.Aii_coords[l][s].end());
-    for (int i = 0; i < nf; ++i) xyz.push_back({H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l});
+// Compares the bottom solve with the reference's 200 sweeps on one random
+// right-hand side (level-0 buffers of the bottom level are scratch here).
+static double bottom_self_check(SigmaDevice& S) {
+    SigmaLevel& B = S.lv[S.levels - 1];
+    const int n = B.n_if;
+    std::vector<float> b(n);
+    uint64_t st = 0x243f6a8885a308d3ULL;
+    for (int i = 0; i < n; ++i) { st = st * 6364136223846793005ULL + 1442695040888963407ULL; b[i] = (float)((double)(st >> 40) / 16777216.0 - 0.5); }
+    hdd_check(hipMemcpy(B.b.p, b.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    S.bottom(0, true);
+    std::vector<float> ud(n), ul(n);
+    hdd_check(hipMemcpy(ud.data(), B.u.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    launch_fill(B.u.p, 0.f, n, 0);
+    for (int i = 0; i < 200; ++i) S.smooth(S.levels - 1, 0);
+    hdd_check(hipMemcpy(ul.data(), B.u.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    double dmax = 0, umax = 0;
+    for (int i = 0; i < n; ++i) { dmax = std::max(dmax, (double)std::fabs(ud[i] - ul[i])); umax = std::max(umax, (double)std::fabs(ul[i])); }
+    launch_fill(B.u.p, 0.f, n, 0);
+    launch_fill(B.b.p, 0.f, n, 0);
+    hdd_check(hipDeviceSynchronize());
+    return umax > 0 ? dmax / umax : dmax;
 }
 
 std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing, int n_mg) {
@@ -396,25 +417,40 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing
         if (l == H.levels - 1 && L.n_if > 0) {
             const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
             const std::string m = mode ? mode : "auto";
+            if (m != "loop" && m != "direct" && m != "auto") throw std::runtime_error("HDD_SIGMA_BOTTOM must be loop, direct or auto");
             if (m != "loop") {
-                S->bottom_rho = estimate_bottom_rho(H, l, fi, fr);
-                // 2% margin on the estimate; 1e-12 is far below float resolution
-                S->bottom_direct = (m == "direct") || std::pow(std::min(1.0, 1.02 * S->bottom_rho), 200.0) < 1e-12;
-            }
-            if (S->bottom_direct) {
                 Csr C; std::vector<Int3> xyz;
                 combined_bottom(H, l, C, xyz);
                 SupernodalFactor fc;
                 factor_spd(C, xyz, 1e-5, fc);
-                S->Lbottom.build({&fc}, {0});
-                S->bot_rhs.alloc(C.rows); S->bot_sol.alloc(C.rows);
-                S->plan_bytes += S->Lbottom.bytes();
+                if (m == "direct") {
+                    S->bottom_direct = true;  // no low-rank correction: exact only when M^200 is negligible
+                } else {
+                    BottomSpectrum sp;
+                    bottom_spectrum(H.Arr[l], 1e-5, fc, L.n_int, kBottomLambdaCut, kBottomBasisBytes, sp);
+                    S->bottom_rho = sp.lambda_max; S->bottom_steps = sp.steps; S->bottom_seconds = sp.seconds;
+                    if (sp.resolved) {
+                        S->bottom_direct = true;
+                        upload_spectrum(*S, sp, L.n_if, H.levels == 1);
+                    }
+                }
+                if (S->bottom_direct) {
+                    S->Lbottom.build({&fc}, {0});
+                    S->bot_rhs.alloc(C.rows); S->bot_sol.alloc(C.rows);
+                    S->plan_bytes += S->Lbottom.bytes();
+                }
             }
         }
         std::vector<SupernodalFactor>().swap(fi);
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
         S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
+    }
+    if (S->bottom_direct) {
+        S->bottom_check = bottom_self_check(*S);
+        const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
+        // the reference's sweeps stay in charge if the shortcut disagrees with them
+        if (!(mode && std::string(mode) == "direct") && !(S->bottom_check < 1e-3)) S->bottom_direct = false;
     }
     S->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (H.levels > 0) { S->b0 = S->lv[0].b.p; S->u0 = S->lv[0].u.p; }
@@ -446,7 +482,24 @@ void SigmaDevice::residual(int l, hipStream_t s) {
                        (const float*)L.vx.p, (const int*)L.Arr.p.p, (const int*)L.Arr.ix.p, (const float*)L.Arr.v.p, (const float*)L.u.p, L.r.p);
 }
 
-void SigmaDevice::vcycle(hipStream_t s) {
+// x* from the combined system [[Aii+eI, Air],[Air^T, Arr+eI]] [y; x*] = [0; b]
+// (the interior part of bot_rhs stays zero), then u = x* + V coef.
+void SigmaDevice::bottom(hipStream_t s, bool u_zero) {
+    SigmaLevel& B = lv[levels - 1];
+    const int ni = B.n_int, n = B.n_if;
+    if (!u_zero && bottom_k && !spec_SV.p) throw std::runtime_error("bottom solve from a nonzero start needs S V");
+    launch_copy(bot_rhs.p + ni, B.b.p, n, s);
+    Lbottom.solve(bot_rhs.p, bot_sol.p, s);
+    if (bottom_k == 0) { launch_copy(B.u.p, bot_sol.p + ni, n, s); return; }
+    hipLaunchKernelGGL(k_spec_dot, dim3(spec_nch, bottom_k), dim3(256), 0, s, n, spec_nch, (const float*)spec_V.p,
+                       (const float*)spec_SV.p, (const float*)B.b.p, u_zero ? (const float*)nullptr : (const float*)B.u.p, spec_part.p);
+    hipLaunchKernelGGL(k_spec_coef, dim3(blocks_for(bottom_k, 64)), dim3(64), 0, s, bottom_k, spec_nch, (const double*)spec_part.p,
+                       (const double*)spec_w1.p, (const double*)spec_w2.p, spec_coef.p);
+    hipLaunchKernelGGL(k_spec_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, bottom_k, (const float*)spec_V.p, (const float*)spec_coef.p,
+                       (const float*)(bot_sol.p + ni), B.u.p);
+}
+
+void SigmaDevice::vcycle(hipStream_t s, bool first) {
     const int NL = levels;
     for (int l = 0; l + 1 < NL; ++l) {
         for (int i = 0; i < smoothing; ++i) smooth(l, s);
@@ -457,16 +510,8 @@ void SigmaDevice::vcycle(hipStream_t s) {
             hipLaunchKernelGGL(k_csr_mv, dim3(blocks_for(C.n_if, 256)), dim3(256), 0, s, C.n_if, lv[l].PT.p.p, lv[l].PT.ix.p, lv[l].PT.v.p,
                                (const float*)lv[l].r.p, C.b.p);
     }
-    if (bottom_direct) {
-        // u = bottom part of [[Aii+eI, Air],[Air^T, Arr+eI]]^{-1} [0; b]; the interior part of bot_rhs stays zero
-        SigmaLevel& B = lv[NL - 1];
-        const int ni = B.n_int;
-        launch_copy(bot_rhs.p + ni, B.b.p, B.n_if, s);
-        Lbottom.solve(bot_rhs.p, bot_sol.p, s);
-        launch_copy(B.u.p, bot_sol.p + ni, B.n_if, s);
-    } else {
-        for (int i = 0; i < 200; ++i) smooth(NL - 1, s);
-    }
+    if (bottom_direct) bottom(s, NL > 1 || first);
+    else for (int i = 0; i < 200; ++i) smooth(NL - 1, s);
     for (int l = NL - 2; l >= 0; --l) {
         SigmaLevel& F = lv[l];
         if (F.n_if)
@@ -479,7 +524,7 @@ void SigmaDevice::vcycle(hipStream_t s) {
 void sigma_vcycles(SigmaDevice& S, hipStream_t s) {
     if (S.levels == 0) return;
     launch_fill(S.u0, 0.f, S.lv[0].u.n, s);
-    for (int i = 0; i < S.n_mg; ++i) S.vcycle(s);
+    for (int i = 0; i < S.n_mg; ++i) S.vcycle(s, i == 0);
 }
 
 void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s) {

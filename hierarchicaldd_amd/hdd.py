@@ -41,7 +41,8 @@ class PcgReport(C.Structure):
 EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
            "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
-           "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats"]
+           "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats",
+           "hdd_bottom_spectrum", "hdd_profile_copy"]
 
 
 def lib():
@@ -75,7 +76,9 @@ def lib():
         L.hdd_pcg_solve.argtypes = [P, C.POINTER(PcgParams), C.POINTER(PcgReport), P, I]
         L.hdd_setup_seconds.argtypes = [P, C.POINTER(C.c_double)]
         L.hdd_profile_sweep.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hdd_profile_copy.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hdd_stats.argtypes = [P, P, I]
+        L.hdd_bottom_spectrum.argtypes = [P, C.c_double, P, I, C.POINTER(I), C.POINTER(C.c_double)]
         for name in EXPORTS:
             if name != "hdd_last_error":
                 getattr(L, name).restype = I
@@ -220,12 +223,25 @@ class DDSolver:
         self._check(lib().hdd_profile_sweep(self.h, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def profile_copy(self, reps: int = 20):
+        """Mean duration (ms) of a plain copy of one level-0 channel and its bytes read plus written."""
+        ms, by = C.c_double(), C.c_double()
+        self._check(lib().hdd_profile_copy(self.h, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
     def stats(self):
-        out = np.zeros(9, dtype=np.float64)
-        self._check(lib().hdd_stats(self.h, _ptr(out), 9))
         keys = ["setup_s", "factor_s", "sigma_device_bytes", "pages", "interface_dofs", "interior_dofs", "subdomains",
-                "bottom_rho", "bottom_direct"]
+                "bottom_rho", "bottom_direct", "bottom_modes", "bottom_eig_s", "bottom_krylov_steps", "bottom_check"]
+        out = np.zeros(len(keys), dtype=np.float64)
+        self._check(lib().hdd_stats(self.h, _ptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
+
+    def bottom_spectrum(self, lambda_cut: float, cap: int = 1024):
+        """Eigenvalues >= lambda_cut of the bottom sigma smoother's iteration matrix (host, double)."""
+        lam = np.zeros(cap, dtype=np.float64)
+        k, lmax = C.c_int(), C.c_double()
+        self._check(lib().hdd_bottom_spectrum(self.h, lambda_cut, _ptr(lam), cap, C.byref(k), C.byref(lmax)))
+        return lam[:min(k.value, cap)].copy(), lmax.value
 
     # reference-facade spellings
     Solve = solve

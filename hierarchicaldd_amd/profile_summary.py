@@ -5,6 +5,11 @@ Usage: python -m hierarchicaldd_amd.profile_summary <rocprof output dir> [--out 
 Reads the rocpd SQLite database (or a *kernel_stats.csv) that
 `rocprofv3 --kernel-trace --stats` writes and prints calls, total and average
 duration and share per kernel, with the command that produced the trace.
+With --by-grid (database input only) each kernel is split by launch grid size,
+which separates the launches of one kernel on differently sized levels.
+With --pmc the input is the *counter_collection.csv of a `rocprofv3 --pmc`
+run (written with --output-format csv): the mean value of each counter per
+kernel and grid size is printed.
 """
 from __future__ import annotations
 
@@ -21,11 +26,25 @@ def short(name: str) -> str:
     return name.split("(")[0]
 
 
-def from_db(path):
+def grid_column(con):
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)").fetchall()]
+    for c in cols:
+        if "grid" in c.lower() and c.lower().endswith("x"):
+            return c
+    raise SystemExit(f"no grid-size column in the kernels view (columns: {cols})")
+
+
+def from_db(path, by_grid=False):
     con = sqlite3.connect(path)
-    rows = con.execute("select name, count(*), sum(duration), avg(duration) from kernels group by name").fetchall()
+    if by_grid:
+        g = grid_column(con)
+        q = f"select name, {g}, count(*), sum(duration), avg(duration) from kernels group by name, {g}"
+        rows = [(f"{short(n)} [grid {gx}]", c, t, a) for n, gx, c, t, a in con.execute(q).fetchall()]
+    else:
+        rows = [(short(n), c, t, a) for n, c, t, a in
+                con.execute("select name, count(*), sum(duration), avg(duration) from kernels group by name").fetchall()]
     con.close()
-    return [(short(n), int(c), float(t) / 1e3, float(a) / 1e3) for n, c, t, a in rows]  # ns -> us
+    return [(n, int(c), float(t) / 1e3, float(a) / 1e3) for n, c, t, a in rows]  # ns -> us
 
 
 def from_csv(path):
@@ -36,15 +55,51 @@ def from_csv(path):
     return out
 
 
+def pmc_rows(path):
+    acc = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            key = (short(r["Kernel_Name"]), int(r["Grid_Size"]), r["Counter_Name"])
+            c, t = acc.get(key, (0, 0.0))
+            acc[key] = (c + 1, t + float(r["Counter_Value"]))
+    return acc
+
+
+def main_pmc(a):
+    files = glob.glob(os.path.join(a.dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {a.dir}")
+    acc = {}
+    for fpath in files:
+        for k, (c, t) in pmc_rows(fpath).items():
+            c0, t0 = acc.get(k, (0, 0.0))
+            acc[k] = (c0 + c, t0 + t)
+    lines = [f"command: {a.cmd}"] if a.cmd else []
+    lines.append(f"{'kernel':48s} {'grid':>10s} {'counter':>12s} {'dispatches':>10s} {'mean value':>14s}")
+    for (n, g, cn), (c, t) in sorted(acc.items(), key=lambda kv: -kv[1][1] / kv[1][0]):
+        lines.append(f"{n[:48]:48s} {g:10d} {cn:>12s} {c:10d} {t / c:14.1f}")
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("dir")
     ap.add_argument("--out")
     ap.add_argument("--cmd", default="")
+    ap.add_argument("--by-grid", action="store_true")
+    ap.add_argument("--top", type=int, default=0, help="print only the N largest rows")
+    ap.add_argument("--pmc", action="store_true")
     a = ap.parse_args()
+    if a.pmc:
+        text = "\n".join(main_pmc(a)) + "\n"
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(text)
+        sys.stdout.write(text)
+        return
     dbs = glob.glob(os.path.join(a.dir, "**", "*_results.db"), recursive=True)
     csvs = glob.glob(os.path.join(a.dir, "**", "*kernel_stats.csv"), recursive=True)
-    rows = from_db(dbs[0]) if dbs else from_csv(csvs[0])
+    rows = from_db(dbs[0], a.by_grid) if dbs else from_csv(csvs[0])
     agg = {}
     for n, c, t, _ in rows:
         c0, t0 = agg.get(n, (0, 0.0))
@@ -54,9 +109,10 @@ def main():
     if a.cmd:
         lines.append(f"command: {a.cmd}")
     lines.append(f"total kernel time: {total / 1e3:.1f} ms over {sum(c for c, _ in agg.values())} launches")
-    lines.append(f"{'kernel':48s} {'calls':>9s} {'total ms':>10s} {'avg us':>9s} {'share':>7s}")
-    for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
-        lines.append(f"{n[:48]:48s} {c:9d} {t / 1e3:10.2f} {t / c:9.2f} {100 * t / total:6.1f}%")
+    lines.append(f"{'kernel':64s} {'calls':>9s} {'total ms':>10s} {'avg us':>9s} {'share':>7s}")
+    ranked = sorted(agg.items(), key=lambda kv: -kv[1][1])
+    for n, (c, t) in ranked[:a.top] if a.top else ranked:
+        lines.append(f"{n[:64]:64s} {c:9d} {t / 1e3:10.2f} {t / c:9.2f} {100 * t / total:6.1f}%")
     text = "\n".join(lines) + "\n"
     if a.out:
         with open(a.out, "w") as f:

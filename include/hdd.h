@@ -146,14 +146,32 @@ hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* seconds);
  * flags, u and b read, u written). No reference counterpart. */
 hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes_per_launch);
 
+/* Measurement: runs `reps` plain copies of one level-0 float channel into the
+ * other smoothing buffer (the calibration kernel for hardware byte counters:
+ * it reads and writes 4 B per page slot); returns the mean duration and the
+ * bytes read plus written by one launch. No reference counterpart. */
+hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes_per_launch);
+
 /* Setup statistics: out[0] setup s, out[1] factorisation s, out[2] device
  * bytes of the interface solver, out[3] level-0 pages, out[4] interface dofs
  * (level 0), out[5] interior dofs (level 0, all subdomains), out[6] subdomains,
- * out[7] estimated contraction of the bottom sigma smoother (-1 if not
- * estimated), out[8] 1 if the bottom level is solved directly (its 200 smooths
- * have converged below float resolution), else 0.
- * n = number of slots provided (up to 9). No reference counterpart. */
+ * out[7] largest eigenvalue of the bottom sigma smoother's iteration matrix
+ * (-1 if not computed), out[8] 1 if the bottom level's 200 smooths are
+ * evaluated as one exact solve plus a low-rank correction, else 0, out[9]
+ * eigenmodes in that correction, out[10] seconds of its eigen-solve, out[11]
+ * block Krylov steps of it (0: dense), out[12] relative max difference of that
+ * evaluation against the 200 smooths on a random right-hand side (-1: not run).
+ * n = number of slots provided (up to 13). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
+
+/* Eigenvalues >= lambda_cut (0 < lambda_cut < 1) of the bottom sigma
+ * smoother's iteration matrix (Arr+eI)^-1 sum_s Air_s^T (Aii_s+eI)^-1 Air_s,
+ * descending, computed on the host in double precision (works on host-only
+ * contexts). *k = their count (only the first cap are written), *lambda_max =
+ * the largest eigenvalue. These are the modes that survive the reference's 200
+ * bottom smooths of SIGMA_MULTIGRID_SOLVER::V_Cycle
+ * (IFS/SIGMA_MULTIGRID_SOLVER.cpp:36-47). No reference counterpart. */
+hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max);
 
 #ifdef __cplusplus
 }

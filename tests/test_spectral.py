@@ -1,0 +1,115 @@
+"""Eigenmodes of the bottom sigma smoother (host, double) against LAPACK / ARPACK.
+
+The bottom of the sigma V-cycle applies u <- S^{-1}(b + K u) 200 times, with
+S = Arr + 1e-5 I and K = sum_s Air_s^T (Aii_s + 1e-5 I)^{-1} Air_s. The library
+evaluates those sweeps as one exact solve plus a correction on the eigenmodes
+of M = S^{-1} K with l^200 >= 1e-9; these tests check the eigenvalues it finds
+(a dense path for bottoms up to 600 dofs, block Lanczos above) against scipy's
+generalized symmetric eigen-solvers on K v = l S v, built from the interface
+operators the library exports.
+
+What the shortcut needs from a mode with eigenvalue l is l^200 to about 1e-7,
+i.e. |dl| * 200 l^199 <= 1e-7: the tests ask exactly that of every kept mode,
+the exact count of modes above the production cut, and the largest eigenvalue
+to 1e-7.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from hierarchicaldd_amd import domain
+
+PRODUCTION_CUT = 1e-9 ** (1.0 / 200.0)  # l^200 = 1e-9
+
+CASES = {
+    # (box n, sub, vlevels, seed, sigma levels)
+    "box32_sub16_L3": (32, 16, 3, 1, 3),   # bottom 513 dofs: dense path, lmax 0.50
+    "box16_sub8_L1": (16, 8, 2, 0, 1),     # 528 dofs: dense path, lmax 0.80
+    "box24_sub8_L1": (24, 8, 2, 0, 1),     # 2591 dofs: block Lanczos, lmax 0.904
+    "box32_sub8_L1": (32, 8, 3, 1, 1),     # 7124 dofs, 64 subdomains: block Lanczos, lmax 0.943
+}
+
+
+def _csr(t):
+    r, c, p, i, v = t
+    return sp.csr_matrix((v.astype(np.float64), i, p), shape=(r, c))
+
+
+def reference_eigenvalues(g, top=40):
+    """Descending eigenvalues of K v = l S v (all of them when small, else the top ones)."""
+    L, nsub = g.sigma_info()
+    l = L - 1
+    n = g.interface_dofs(l)
+    S = (_csr(g.matrix("Arr", l)) + 1e-5 * sp.eye(n)).tocsc()
+    blocks = []
+    for s in range(nsub):
+        Aii = _csr(g.matrix("Aii", l, s))
+        if Aii.shape[0]:
+            blocks.append((spl.splu((Aii + 1e-5 * sp.eye(Aii.shape[0])).tocsc()), _csr(g.matrix("Air", l, s))))
+
+    def K(u):
+        out = np.zeros(n)
+        for lu, Air in blocks:
+            out += Air.T @ lu.solve(Air @ u)
+        return out
+
+    if n <= 3000:
+        Kd = np.column_stack([K(e) for e in np.eye(n)])
+        Sd = S.toarray()
+        return sla.eigh(0.5 * (Kd + Kd.T), 0.5 * (Sd + Sd.T), eigvals_only=True)[::-1]
+    Kop = spl.LinearOperator((n, n), matvec=K, dtype=np.float64)
+    Slu = spl.splu(S)
+    Sinv = spl.LinearOperator((n, n), matvec=Slu.solve, dtype=np.float64)
+    w = spl.eigsh(Kop, k=top, M=S, Minv=Sinv, which="LA", tol=1e-12, return_eigenvectors=False)
+    return np.sort(w)[::-1]
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request):
+    from hierarchicaldd_amd import hdd
+    n, sub, vl, seed, sl = CASES[request.param]
+    dom, cfg = domain.small_box(n, sub, vl, seed=seed, sigma_levels=sl)
+    g = hdd.DDSolver(dom, cfg, host_only=True)
+    w = reference_eigenvalues(g)
+    yield request.param, g, w
+    g.close()
+
+
+def test_largest_eigenvalue(case):
+    name, g, w = case
+    _, lmax = g.bottom_spectrum(PRODUCTION_CUT)
+    assert abs(lmax - w[0]) < 1e-7, (name, lmax, w[0])
+
+
+def test_count_above_production_cut(case):
+    name, g, w = case
+    lam, _ = g.bottom_spectrum(PRODUCTION_CUT)
+    assert not np.any(np.abs(w - PRODUCTION_CUT) < 1e-6), "reference eigenvalue too close to the cut"
+    assert len(lam) == int((w >= PRODUCTION_CUT).sum()), (name, len(lam), w[:8])
+
+
+@pytest.mark.parametrize("cut", [0.5, PRODUCTION_CUT])
+def test_kept_modes_accurate_where_they_matter(case, cut):
+    name, g, w = case
+    lam, _ = g.bottom_spectrum(cut)
+    expect = w[w >= cut]
+    k = min(len(lam), len(expect))
+    err = np.abs(lam[:k] - expect[:k]) * 200.0 * expect[:k] ** 199
+    assert np.all(err <= 1e-7), (name, lam[:k], expect[:k], err)
+    assert np.all(np.diff(lam) <= 0) and np.all((lam >= cut) & (lam < 1.0))
+
+
+def test_spectrum_inside_unit_interval(case):
+    """K positive semi-definite and S - K positive definite: eigenvalues in [0, 1)."""
+    _, _, w = case
+    assert w[-1] > -1e-10 and w[0] < 1.0
+
+
+def test_rejects_bad_cut(case):
+    from hierarchicaldd_amd import hdd
+    _, g, _ = case
+    for bad in (0.0, 1.0, -0.5):
+        with pytest.raises(hdd.HddError):
+            g.bottom_spectrum(bad)
