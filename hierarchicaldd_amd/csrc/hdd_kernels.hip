@@ -36,17 +36,20 @@ __device__ __forceinline__ int nb_index(const int* __restrict__ nbr, int p, int 
     }
 }
 
-// float residual of one cell, reference operation order
+// float residual of one cell, reference operation order. The six neighbour
+// values are loaded unconditionally (the zero page backs missing pages, so
+// every address is valid) and selected by the face bits afterwards: the loads
+// then leave together with the cell's own, instead of waiting for its flags.
 __device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int p, int c, uint32_t f,
                                                const float* __restrict__ u, float b) {
     const float cv = u[p * 256 + c];
+    float nv[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) nv[k] = u[nb_index(nbr, p, c, k)];
     float result = 0.f;
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-        if (f & (kFaceMX << k)) {
-            const float nv = u[nb_index(nbr, p, c, k)];
-            result -= (nv - cv);
-        }
+        if (f & (kFaceMX << k)) result -= (nv[k] - cv);
     return b - result;
 }
 
@@ -59,11 +62,8 @@ __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restr
     const uint32_t sel = f & (kActive | kBoundary);
     const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
     float u = u_in[e];
-    const float bb = b[e];
-    if (w) {
-        const float r = cell_residual(L.nbr, p, c, f, u_in, bb);
-        u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * r;
-    }
+    const float r = cell_residual(L.nbr, p, c, f, u_in, b[e]);
+    if (w) u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * r;
     u_out[e] = u;
 }
 

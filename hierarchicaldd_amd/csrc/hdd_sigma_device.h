@@ -6,6 +6,7 @@
 #include "hdd_factor.h"
 #include "hdd_runtime.h"
 #include "hdd_sigma.h"
+#include "hdd_triplan.h"
 
 namespace hdd {
 
@@ -18,14 +19,15 @@ struct CsrDev {
 };
 
 // Exact solve x = (A + eps I)^{-1} b for a batch of independent matrices,
-// from their supernodal factors (see hdd_sigma_device.hip for the layout).
+// from their supernodal factors (plan layout in hdd_triplan.h).
 struct TriPlan {
     int n = 0, H = -1;
-    std::vector<int> lev_begin;                 // rows of height h: rowlist[lev_begin[h] .. lev_begin[h+1])
-    std::vector<int> lanes;                     // lanes per row, [height][fwd gather, fwd diag, bwd gather, bwd diag]
-    DBuf<int> orig_of, sn_of, s0, sz, nr, roff, srows, rowlist, lp, li;
-    DBuf<long long> doff, loff;
-    DBuf<float> diag, diagT, l21, lv, y, t;
+    size_t factor_nnz = 0;
+    // per-height ranges and launch parameters (host copies)
+    std::vector<int> sl_begin, sl_rows, row_begin, ci_begin, bi_begin, max_sz, big, G, Gc, Gf, Gb;
+    DBuf<int> orig_of, s0, sz, nr, roff, srows, sn_of, sl_sn, sl_r0, rowlist, inc_ptr, ci_sn, ci_a0, ci_a1, bi_sn, bi_c, bi_j0;
+    DBuf<long long> loff, doff, cofs, cpos, bofs;
+    DBuf<float> diag, diagT, l21, c, bpart, t, y;
     void build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets);
     void solve(const float* b_orig, float* x_orig, hipStream_t s);
     size_t bytes() const;

@@ -5,14 +5,13 @@
 //   SIGMA_SMOOTHER::Compute_Residual  IFS/SIGMA_SMOOTHER.cpp:328-352
 //
 // Exact solves with (A + 1e-5 I) use supernodal Cholesky factors (host,
-// double) turned into float "triangular-solve plans": per supernode the
-// explicit inverse of its diagonal block and its dense sub-diagonal block,
-// plus a row-wise CSR of the sub-diagonal part for the forward gather. A solve
+// double) turned into float triangular-solve plans (hdd_triplan.h). A solve
 // walks the supernode tree by height (leaves first forward, root first
-// backward); every row of a height level is one 64-lane wavefront, so a level
-// is two launches: a gather of the off-diagonal products and a dense product
-// with the diagonal-block inverse. All subdomains' Aii of one sigma level form
-// one batch, so each level of all their trees is a single launch.
+// backward); each height is two launches: wavefronts over bounded gather
+// items of the sub-diagonal blocks, then workgroups over row slices of each
+// supernode that add the items' partial sums and apply the explicit inverse
+// of the diagonal block from LDS. All subdomains' Aii of one sigma level form
+// one batch, so each height of all their trees is one pair of launches.
 //
 // Sparse products keep Eigen's accumulation order (rows in column order, the
 // transposed products in source-row order), built as gathers so no atomics
@@ -35,10 +34,6 @@ namespace hdd {
 
 namespace {
 
-// Every kernel below gives one row to a group of G lanes (G = 4..64, a power of
-// two chosen per height level from the level's mean row length), so a 64-lane
-// wavefront works on 64/G rows at once; the group reduces with shuffles of
-// width G.
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
@@ -46,62 +41,155 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+// forward reduce (heights with big supernodes): t_i = b_i - sum of the
+// contributions at row i's incidences, G lanes per row
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_fwd_gather(const int* __restrict__ rows, int nrows, const int* __restrict__ orig_of,
-                                                        const float* __restrict__ b, const int* __restrict__ lp, const int* __restrict__ li,
-                                                        const float* __restrict__ lv, const float* __restrict__ y, float* __restrict__ t) {
+__global__ void __launch_bounds__(256) k_tri_fwd_reduce(const int* __restrict__ rows, int nrows, const int* __restrict__ inc_ptr,
+                                                        const long long* __restrict__ cpos, const float* __restrict__ c,
+                                                        const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ t) {
     const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w];
     float s = 0.f;
-    for (int e = lp[i] + lane; e < lp[i + 1]; e += G) s += lv[e] * y[li[e]];
+    for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += G) s += c[cpos[p]];
     s = group_sum<G>(s);
     if (lane == 0) t[i] = b[orig_of[i]] - s;
 }
 
+// forward diagonal step: a workgroup per row slice [r0, r0 + slice_rows) of a
+// supernode. t of rows [0, rend) into LDS (from the reduce kernel's output on
+// big heights, else summed here), then y[r] = sum_{k<=r} Linv(r,k) t[k] with
+// G lanes per row.
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_fwd_diag(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
-                                                      const int* __restrict__ s0, const int* __restrict__ sz, const long long* __restrict__ doff,
-                                                      const float* __restrict__ diag, const float* __restrict__ t, float* __restrict__ y) {
-    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
-    if (w >= nrows) return;
-    const int i = rows[w], s = sn_of[i], base = s0[s], n = sz[s], r = i - base;
-    const float* D = diag + doff[s] + (long long)r * n;
-    float acc = 0.f;
-    for (int k = lane; k <= r; k += G) acc += D[k] * t[base + k];
-    acc = group_sum<G>(acc);
-    if (lane == 0) y[i] = acc;
+__global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
+                                                    const int* __restrict__ s0, const int* __restrict__ sz,
+                                                    const long long* __restrict__ doff, const float* __restrict__ diag,
+                                                    const float* __restrict__ tg, const int* __restrict__ inc_ptr,
+                                                    const long long* __restrict__ cpos, const float* __restrict__ c,
+                                                    const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ y) {
+    extern __shared__ float ts[];
+    const int g = sl_sn[q0 + blockIdx.x], r0 = sl_r0[q0 + blockIdx.x], n = sz[g], base = s0[g];
+    const int rend = min(n, r0 + slice_rows);
+    for (int k = threadIdx.x; k < rend; k += 256) {
+        const int i = base + k;
+        if (tg) {
+            ts[k] = tg[i];
+        } else {
+            float sum = 0.f;
+            for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
+            ts[k] = b[orig_of[i]] - sum;
+        }
+    }
+    __syncthreads();
+    const float* D = diag + doff[g];
+    const int lane = threadIdx.x & (G - 1);
+    for (int r = r0 + (int)threadIdx.x / G; r < rend; r += 256 / G) {
+        const float* Dr = D + (long long)r * n;
+        float acc = 0.f;
+        for (int k = lane; k <= r; k += G) acc += Dr[k] * ts[k];
+        acc = group_sum<G>(acc);
+        if (lane == 0) y[base + r] = acc;
+    }
 }
 
+// contributions c_g[a] = L21_g[a, :] . y_g for the rows [a0, a1) of an item, G lanes per row
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_bwd_gather(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
-                                                        const int* __restrict__ s0, const int* __restrict__ nr, const long long* __restrict__ loff,
-                                                        const int* __restrict__ roff, const int* __restrict__ srows, const float* __restrict__ l21,
+__global__ void __launch_bounds__(256) k_tri_fwd_contrib(int i0, int i1, const int* __restrict__ ci_sn, const int* __restrict__ ci_a0,
+                                                         const int* __restrict__ ci_a1, const int* __restrict__ s0, const int* __restrict__ sz,
+                                                         const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                         const float* __restrict__ l21, const float* __restrict__ y, float* __restrict__ c) {
+    // a workgroup per item; its rows go to the 256/G lane groups in turn
+    const int it = i0 + blockIdx.x;
+    if (it >= i1) return;
+    const int g = ci_sn[it], n = sz[g], a1 = ci_a1[it];
+    const float* L = l21 + loff[g];
+    const float* yy = y + s0[g];
+    float* cg = c + cofs[g];
+    const int lane = threadIdx.x & (G - 1);
+    for (int a = ci_a0[it] + (int)threadIdx.x / G; a < a1; a += 256 / G) {
+        const float* La = L + (long long)a * n;
+        float acc = 0.f;
+        for (int j = lane; j < n; j += G) acc += La[j] * yy[j];
+        acc = group_sum<G>(acc);
+        if (lane == 0) cg[a] = acc;
+    }
+}
+
+// backward tiles: one wavefront per (supernode g, row chunk cc, column block j0);
+// lane j accumulates L21_g[a, j] z[rows_g[a]] over the chunk's rows (z: the
+// ancestors' backward results, final)
+__global__ void __launch_bounds__(256) k_tri_bwd_tiles(int i0, int i1, const int* __restrict__ bi_sn, const int* __restrict__ bi_c,
+                                                       const int* __restrict__ bi_j0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                                       const long long* __restrict__ loff, const int* __restrict__ roff,
+                                                       const int* __restrict__ srows, const long long* __restrict__ bofs,
+                                                       const float* __restrict__ l21, const float* __restrict__ z, float* __restrict__ bpart) {
+    const int it = i0 + blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (it >= i1) return;
+    const int g = bi_sn[it], n = sz[g], cc = bi_c[it], j = bi_j0[it] + lane;
+    const int a0 = cc * TriPlanHost::kTile, m = min(nr[g] - a0, TriPlanHost::kTile);
+    // lane q holds z at the tile's row q; the row loop then issues only independent L21 loads
+    const float zq = lane < m ? z[srows[roff[g] + a0 + lane]] : 0.f;
+    const float* L = l21 + loff[g] + (long long)a0 * n + min(j, n - 1);
+    float acc = 0.f;
+#pragma unroll 8
+    for (int q = 0; q < m; ++q) acc += L[(long long)q * n] * __shfl(zq, q);
+    if (j < n) bpart[bofs[g] + (long long)cc * n + j] = acc;
+}
+
+// backward reduce (heights with big supernodes): t_j = y_j - sum over the row
+// chunks of the tile partials of column j of its supernode, G lanes per row
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_bwd_reduce(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
+                                                        const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                                        const long long* __restrict__ bofs, const float* __restrict__ bpart,
                                                         const float* __restrict__ y, float* __restrict__ t) {
     const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
-    const int i = rows[w], s = sn_of[i], r = i - s0[s], m = nr[s];
-    const float* Lc = l21 + loff[s] + (long long)r * m;
-    const int* R = srows + roff[s];
-    float acc = 0.f;
-    for (int a = lane; a < m; a += G) acc += Lc[a] * y[R[a]];
-    acc = group_sum<G>(acc);
-    if (lane == 0) t[i] = y[i] - acc;
+    const int i = rows[w], g = sn_of[i], n = sz[g], k = i - s0[g];
+    const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
+    const float* P = bpart + bofs[g] + k;
+    float s = 0.f;
+    for (int cc = lane; cc < nch; cc += G) s += P[(long long)cc * n];
+    s = group_sum<G>(s);
+    if (lane == 0) t[i] = y[i] - s;
 }
 
+// backward diagonal step: t of rows [r0, n) into LDS (reduce output on big
+// heights, else summed here), then z[r] = sum_{k>=r} Linv(k,r) t[k]; y holds
+// the forward result and z the backward one (separate arrays: the slices of
+// one supernode read rows that other slices write); also writes x in the
+// original numbering
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_bwd_diag(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
-                                                      const int* __restrict__ s0, const int* __restrict__ sz, const long long* __restrict__ doff,
-                                                      const float* __restrict__ diagT, const float* __restrict__ t, float* __restrict__ y,
-                                                      const int* __restrict__ orig_of, float* __restrict__ out) {
-    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
-    if (w >= nrows) return;
-    const int i = rows[w], s = sn_of[i], base = s0[s], n = sz[s], r = i - base;
-    const float* D = diagT + doff[s] + (long long)r * n;
-    float acc = 0.f;
-    for (int k = r + lane; k < n; k += G) acc += D[k] * t[base + k];
-    acc = group_sum<G>(acc);
-    if (lane == 0) { y[i] = acc; out[orig_of[i]] = acc; }
+__global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
+                                                    const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                                    const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                    const float* __restrict__ tg, const long long* __restrict__ bofs,
+                                                    const float* __restrict__ bpart, const int* __restrict__ orig_of,
+                                                    const float* __restrict__ y, float* __restrict__ z, float* __restrict__ x) {
+    extern __shared__ float ts[];
+    const int g = sl_sn[q0 + blockIdx.x], r0 = sl_r0[q0 + blockIdx.x], n = sz[g], base = s0[g];
+    const int rend = min(n, r0 + slice_rows);
+    const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
+    for (int k = r0 + (int)threadIdx.x; k < n; k += 256) {
+        if (tg) {
+            ts[k] = tg[base + k];
+        } else {
+            const float* P = bpart + bofs[g] + k;
+            float sum = 0.f;
+            for (int cc = 0; cc < nch; ++cc) sum += P[(long long)cc * n];
+            ts[k] = y[base + k] - sum;
+        }
+    }
+    __syncthreads();
+    const float* DT = diagT + doff[g];
+    const int lane = threadIdx.x & (G - 1);
+    for (int r = r0 + (int)threadIdx.x / G; r < rend; r += 256 / G) {
+        const float* Dr = DT + (long long)r * n;
+        float acc = 0.f;
+        for (int k = r + lane; k < n; k += G) acc += Dr[k] * ts[k];
+        acc = group_sum<G>(acc);
+        if (lane == 0) { z[base + r] = acc; x[orig_of[base + r]] = acc; }
+    }
 }
 
 // y = A x for CSR A (one thread per row, column order)
@@ -205,128 +293,84 @@ static Csr transpose(const Csr& A) {
 }
 
 void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets) {
-    n = 0; H = -1;
-    for (size_t m = 0; m < F.size(); ++m) { n = std::max(n, offsets[m] + F[m]->n); H = std::max(H, F[m]->n ? F[m]->max_height : -1); }
+    TriPlanHost P;
+    build_triplan(F, offsets, P);
+    n = P.n; H = P.H; factor_nnz = P.factor_nnz;
     if (n == 0) return;
-    std::vector<int> h_orig(n), h_snof(n);
-    std::vector<int> h_s0, h_sz, h_nr, h_roff, h_srows, h_height;
-    std::vector<long long> h_doff, h_loff;
-    long long dsize = 0, lsize = 0;
-    struct Ref { int m, s; };
-    std::vector<Ref> refs;
-    for (size_t m = 0; m < F.size(); ++m) {
-        const SupernodalFactor& f = *F[m];
-        const int off = offsets[m];
-        for (int k = 0; k < f.n; ++k) h_orig[off + k] = off + f.perm[k];
-        for (size_t s = 0; s < f.sn.size(); ++s) {
-            const Supernode& S = f.sn[s];
-            const int gid = (int)h_s0.size();
-            h_s0.push_back(off + S.s0); h_sz.push_back(S.n); h_nr.push_back((int)S.rows.size());
-            h_roff.push_back((int)h_srows.size());
-            for (int r : S.rows) h_srows.push_back(off + r);
-            h_doff.push_back(dsize); dsize += (long long)S.n * S.n;
-            h_loff.push_back(lsize); lsize += (long long)S.n * S.rows.size();
-            h_height.push_back(S.height);
-            for (int k = 0; k < S.n; ++k) h_snof[off + S.s0 + k] = gid;
-            refs.push_back({(int)m, (int)s});
-        }
-    }
-    const int nsn = (int)h_s0.size();
-    std::vector<float> h_diag(dsize), h_diagT(dsize), h_l21(lsize);
-    std::vector<int> lcount(n + 1, 0);
-    for (int g = 0; g < nsn; ++g)
-        for (int a = 0; a < h_nr[g]; ++a) lcount[h_srows[h_roff[g] + a] + 1] += h_sz[g];
-    for (int i = 0; i < n; ++i) lcount[i + 1] += lcount[i];
-    std::vector<int> h_li(lcount[n]);
-    std::vector<float> h_lv(lcount[n]);
-#pragma omp parallel for schedule(dynamic)
-    for (int g = 0; g < nsn; ++g) {
-        const SupernodalFactor& f = *F[refs[g].m];
-        const int s = refs[g].s, ns = h_sz[g], nr = h_nr[g];
-        std::vector<double> inv(f.L11[s]);
-        invert_lower(inv, ns);
-        float* D = &h_diag[h_doff[g]];
-        float* DT = &h_diagT[h_doff[g]];
-        for (int r = 0; r < ns; ++r)
-            for (int k = 0; k < ns; ++k) {
-                D[(size_t)r * ns + k] = k <= r ? (float)inv[(size_t)k * ns + r] : 0.f;   // Linv(r,k)
-                DT[(size_t)r * ns + k] = k >= r ? (float)inv[(size_t)r * ns + k] : 0.f;  // Linv(k,r)
-            }
-        float* L = &h_l21[h_loff[g]];
-        for (size_t e = 0; e < (size_t)ns * nr; ++e) L[e] = (float)f.L21[s][e];
-    }
-    // forward gather rows: supernodes in order give ascending columns per row
-    {
-        std::vector<int> fill(lcount.begin(), lcount.end() - 1);
-        for (int g = 0; g < nsn; ++g) {
-            const int ns = h_sz[g], nr = h_nr[g];
-            for (int j = 0; j < ns; ++j)
-                for (int a = 0; a < nr; ++a) {
-                    const int row = h_srows[h_roff[g] + a];
-                    const int k = fill[row]++;
-                    h_li[k] = h_s0[g] + j;
-                    h_lv[k] = h_l21[h_loff[g] + (size_t)j * nr + a];
-                }
-        }
-    }
-    std::vector<int> h_rows;
-    lev_begin.assign(H + 2, 0);
-    for (int h = 0; h <= H; ++h) {
-        lev_begin[h] = (int)h_rows.size();
-        for (int g = 0; g < nsn; ++g)
-            if (h_height[g] == h) for (int k = 0; k < h_sz[g]; ++k) h_rows.push_back(h_s0[g] + k);
-    }
-    lev_begin[H + 1] = (int)h_rows.size();
-    // lanes per row for each level and phase, from the mean work per row
-    auto lanes_for = [](double len) { int g = 4; while (g < 64 && g * 3 < len) g *= 2; return g; };
-    lanes.assign((size_t)(H + 1) * 4, 4);
-    for (int h = 0; h <= H; ++h) {
-        double fg = 0, fd = 0, bg = 0, bd = 0;
-        const int cnt = lev_begin[h + 1] - lev_begin[h];
-        if (!cnt) continue;
-        for (int q = lev_begin[h]; q < lev_begin[h + 1]; ++q) {
-            const int i = h_rows[q], g = h_snof[i], r = i - h_s0[g];
-            fg += lcount[i + 1] - lcount[i]; fd += r + 1; bg += h_nr[g]; bd += h_sz[g] - r;
-        }
-        lanes[h * 4 + 0] = lanes_for(fg / cnt); lanes[h * 4 + 1] = lanes_for(fd / cnt);
-        lanes[h * 4 + 2] = lanes_for(bg / cnt); lanes[h * 4 + 3] = lanes_for(bd / cnt);
-    }
-    orig_of.upload(h_orig); sn_of.upload(h_snof); s0.upload(h_s0); sz.upload(h_sz); nr.upload(h_nr);
-    roff.upload(h_roff); srows.upload(h_srows); rowlist.upload(h_rows); lp.upload(lcount); li.upload(h_li);
-    doff.upload(h_doff); loff.upload(h_loff);
-    diag.upload(h_diag); diagT.upload(h_diagT); l21.upload(h_l21); lv.upload(h_lv);
-    y.alloc(n); t.alloc(n);
+    sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
+    max_sz = P.max_sz; big = P.big; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
+    orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
+    sn_of.upload(P.sn_of); sl_sn.upload(P.sl_sn); sl_r0.upload(P.sl_r0); rowlist.upload(P.rowlist); inc_ptr.upload(P.inc_ptr);
+    ci_sn.upload(P.ci_sn); ci_a0.upload(P.ci_a0); ci_a1.upload(P.ci_a1); bi_sn.upload(P.bi_sn); bi_c.upload(P.bi_c); bi_j0.upload(P.bi_j0);
+    loff.upload(P.loff); doff.upload(P.doff); cofs.upload(P.cofs); cpos.upload(P.cpos); bofs.upload(P.bofs);
+    diag.upload(P.diag); diagT.upload(P.diagT); l21.upload(P.l21);
+    c.alloc(std::max<long long>(1, P.c_size)); bpart.alloc(std::max<long long>(1, P.bpart_size));
+    t.alloc(n); y.alloc(2 * (size_t)n);  // forward result, then backward result
 }
 
 size_t TriPlan::bytes() const {
-    return (orig_of.n + sn_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + rowlist.n + lp.n + li.n) * 4 + (doff.n + loff.n) * 8 +
-           (diag.n + diagT.n + l21.n + lv.n + y.n + t.n) * 4;
+    return (orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
+            ci_a1.n + bi_sn.n + bi_c.n + bi_j0.n) * 4 +
+           (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4;
 }
 
-#define HDD_LANES_DISPATCH(G, KERNEL, GRIDROWS, ...)                                                     \
-    switch (G) {                                                                                        \
-        case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(blocks_for(GRIDROWS * 4, 256)), dim3(256), 0, __VA_ARGS__); break;   \
-        case 8: hipLaunchKernelGGL(KERNEL<8>, dim3(blocks_for(GRIDROWS * 8, 256)), dim3(256), 0, __VA_ARGS__); break;   \
-        case 16: hipLaunchKernelGGL(KERNEL<16>, dim3(blocks_for(GRIDROWS * 16, 256)), dim3(256), 0, __VA_ARGS__); break; \
-        case 32: hipLaunchKernelGGL(KERNEL<32>, dim3(blocks_for(GRIDROWS * 32, 256)), dim3(256), 0, __VA_ARGS__); break; \
-        default: hipLaunchKernelGGL(KERNEL<64>, dim3(blocks_for(GRIDROWS * 64, 256)), dim3(256), 0, __VA_ARGS__); break; \
+#define HDD_SN_DISPATCH(G, KERNEL, BLOCKS, SHMEM, ...)                                                           \
+    switch (G) {                                                                                                \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;              \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;              \
+        case 16: hipLaunchKernelGGL(KERNEL<16>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;            \
+        case 32: hipLaunchKernelGGL(KERNEL<32>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;            \
+        default: hipLaunchKernelGGL(KERNEL<64>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;            \
+    }
+
+#define HDD_ROWS_DISPATCH(G, KERNEL, ROWS, ...)                                                                \
+    switch (G) {                                                                                                \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, dim3(blocks_for((ROWS) * 4, 256)), dim3(256), 0, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, dim3(blocks_for((ROWS) * 8, 256)), dim3(256), 0, __VA_ARGS__); break;   \
+        case 16: hipLaunchKernelGGL(KERNEL<16>, dim3(blocks_for((ROWS) * 16, 256)), dim3(256), 0, __VA_ARGS__); break; \
+        case 32: hipLaunchKernelGGL(KERNEL<32>, dim3(blocks_for((ROWS) * 32, 256)), dim3(256), 0, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(KERNEL<64>, dim3(blocks_for((ROWS) * 64, 256)), dim3(256), 0, __VA_ARGS__); break; \
     }
 
 void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     if (n == 0) return;
+    float* yf = y.p;
+    float* zb = y.p + n;
     for (int h = 0; h <= H; ++h) {
-        const int cnt = lev_begin[h + 1] - lev_begin[h];
-        if (!cnt) continue;
-        const int* rows = rowlist.p + lev_begin[h];
-        HDD_LANES_DISPATCH(lanes[h * 4 + 0], k_tri_fwd_gather, cnt, s, rows, cnt, orig_of.p, b, lp.p, li.p, lv.p, y.p, t.p);
-        HDD_LANES_DISPATCH(lanes[h * 4 + 1], k_tri_fwd_diag, cnt, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diag.p, t.p, y.p);
+        const int nrows = row_begin[h + 1] - row_begin[h];
+        const int* rows = rowlist.p + row_begin[h];
+        if (big[h] && nrows)
+            HDD_ROWS_DISPATCH(Gf[h], k_tri_fwd_reduce, nrows, s, rows, nrows, (const int*)inc_ptr.p, (const long long*)cpos.p,
+                              (const float*)c.p, (const int*)orig_of.p, b, t.p);
+        const int q0 = sl_begin[h], nq = sl_begin[h + 1] - sl_begin[h];
+        if (nq)
+            HDD_SN_DISPATCH(G[h], k_tri_fwd_sn, nq, (size_t)max_sz[h] * sizeof(float), s, q0, sl_rows[h], (const int*)sl_sn.p, (const int*)sl_r0.p,
+                            (const int*)s0.p, (const int*)sz.p, (const long long*)doff.p, (const float*)diag.p,
+                            big[h] ? (const float*)t.p : (const float*)nullptr, (const int*)inc_ptr.p, (const long long*)cpos.p,
+                            (const float*)c.p, (const int*)orig_of.p, b, yf);
+        const int i0 = ci_begin[h], ni = ci_begin[h + 1] - ci_begin[h];
+        if (ni)
+            HDD_SN_DISPATCH(Gc[h], k_tri_fwd_contrib, ni, 0, s, i0, i0 + ni, (const int*)ci_sn.p, (const int*)ci_a0.p, (const int*)ci_a1.p,
+                            (const int*)s0.p, (const int*)sz.p, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p,
+                            (const float*)yf, c.p);
     }
     for (int h = H; h >= 0; --h) {
-        const int cnt = lev_begin[h + 1] - lev_begin[h];
-        if (!cnt) continue;
-        const int* rows = rowlist.p + lev_begin[h];
-        HDD_LANES_DISPATCH(lanes[h * 4 + 2], k_tri_bwd_gather, cnt, s, rows, cnt, sn_of.p, s0.p, nr.p, loff.p, roff.p, srows.p, l21.p, y.p, t.p);
-        HDD_LANES_DISPATCH(lanes[h * 4 + 3], k_tri_bwd_diag, cnt, s, rows, cnt, sn_of.p, s0.p, sz.p, doff.p, diagT.p, t.p, y.p, orig_of.p, x);
+        const int i0 = bi_begin[h], ni = bi_begin[h + 1] - bi_begin[h];
+        if (ni)
+            hipLaunchKernelGGL(k_tri_bwd_tiles, dim3(blocks_for(ni, 4)), dim3(256), 0, s, i0, i0 + ni, (const int*)bi_sn.p, (const int*)bi_c.p,
+                               (const int*)bi_j0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)loff.p, (const int*)roff.p,
+                               (const int*)srows.p, (const long long*)bofs.p, (const float*)l21.p, (const float*)zb, bpart.p);
+        const int nrows = row_begin[h + 1] - row_begin[h];
+        const int* rows = rowlist.p + row_begin[h];
+        if (big[h] && nrows)
+            HDD_ROWS_DISPATCH(Gb[h], k_tri_bwd_reduce, nrows, s, rows, nrows, (const int*)sn_of.p, (const int*)s0.p, (const int*)sz.p,
+                              (const int*)nr.p, (const long long*)bofs.p, (const float*)bpart.p, (const float*)yf, t.p);
+        const int q0 = sl_begin[h], nq = sl_begin[h + 1] - sl_begin[h];
+        if (nq)
+            HDD_SN_DISPATCH(G[h], k_tri_bwd_sn, nq, (size_t)max_sz[h] * sizeof(float), s, q0, sl_rows[h], (const int*)sl_sn.p, (const int*)sl_r0.p,
+                            (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diagT.p,
+                            big[h] ? (const float*)t.p : (const float*)nullptr, (const long long*)bofs.p, (const float*)bpart.p,
+                            (const int*)orig_of.p, (const float*)yf, zb, x);
     }
 }
 

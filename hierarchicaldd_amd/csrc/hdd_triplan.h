@@ -1,0 +1,67 @@
+// Host construction of the GPU triangular-solve plan for a batch of supernodal
+// factors: x = (A + eps I)^{-1} b = L^{-T} L^{-1} b for every factor of the
+// batch at once (the exact solves of SIGMA_SMOOTHER::Smooth and
+// Compute_Residual, IFS/SIGMA_SMOOTHER.cpp:292-352, which the reference hands
+// to PARDISO).
+//
+// Layout. Supernodes of all factors are numbered globally and grouped by tree
+// height (leaves 0). Per supernode g: the explicit inverse of its diagonal
+// block, row-major (diag, lower triangle used) and transposed (diagT, upper
+// triangle used), and its sub-diagonal block L21_g row-major (nr x n), the
+// only copy of L21. Both passes read L21 along its rows, contiguously.
+//
+// Forward, per height h:
+//   reduce  t_i = b_i - sum over the incidences of row i of c[pos]       (heights with big supernodes)
+//   diag    y_g = Linv_g t_g, a workgroup per row slice                  (t inline for small supernodes)
+//   contrib c_g = L21_g y_g, one value per sub-diagonal row of g         (consumed by ancestors' rows)
+// Backward, per height h from the root:
+//   tiles   p_g[c, j] = sum_{a in row chunk c} L21_g[a, j] z[rows_g[a]]  (64 x 64 tiles, one wavefront each)
+//   reduce  t_j = y_j - sum_c p_g[c, j]                                   (heights with big supernodes)
+//   diag    z_g = Linv_g^T t_g
+// Every sum runs in a fixed order, so results are deterministic.
+#pragma once
+#include <vector>
+#include "hdd_factor.h"
+
+namespace hdd {
+
+struct TriPlanHost {
+    static constexpr int kSliceRows = 128;       // rows of Linv t per workgroup (a supernode up to this size is one slice)
+    static constexpr int kBigSliceRows = 16;     // rows per workgroup for heights with bigger supernodes
+    static constexpr int kTile = 64;             // backward tile: rows x columns
+    static constexpr int kContribEntries = 4096; // L21 entries per contribution item
+    static constexpr int kMaxSupernode = 16384;  // the per-supernode kernels keep t in LDS
+    int n = 0, H = -1, nsn = 0;
+    std::vector<int> orig_of;                 // plan row -> row of the batch's original numbering
+    std::vector<int> s0, sz, nr, roff, srows; // supernode first row, size, sub-diagonal rows (global plan rows)
+    std::vector<int> sn_of;                   // plan row -> supernode
+    std::vector<long long> loff, doff;        // offsets into l21 / diag
+    std::vector<float> diag, diagT, l21;
+    std::vector<int> sn_begin, max_sz, big;   // per height: supernode range in sn_list, largest supernode, 1 if > kSliceRows
+    std::vector<int> sn_list;
+    std::vector<int> sl_sn, sl_r0, sl_begin;  // row slices (supernode, first row) by height
+    std::vector<int> sl_rows;                 // rows per slice, per height
+    std::vector<int> rowlist, row_begin;      // plan rows by height (for the reduce kernels)
+    std::vector<int> G;                       // lanes per row of the diag kernels, per height
+    // forward
+    std::vector<long long> cofs;              // per supernode: offset of its contributions c_g
+    std::vector<int> inc_ptr;                 // per plan row: incidences [inc_ptr[i], inc_ptr[i+1])
+    std::vector<long long> cpos;              // incidence -> position in c (ascending supernode)
+    std::vector<int> ci_sn, ci_a0, ci_a1, ci_begin, Gc;  // contribution items by height, lanes per row
+    std::vector<int> Gf;                      // lanes per row of the forward reduce, per height
+    long long c_size = 0;
+    // backward
+    std::vector<long long> bofs;              // per supernode: offset of its tile partials (nchunk x n)
+    std::vector<int> bi_sn, bi_c, bi_j0, bi_begin;      // tiles by height
+    std::vector<int> Gb;                      // lanes per row of the backward reduce, per height
+    long long bpart_size = 0;
+    size_t factor_nnz = 0;
+};
+
+void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P);
+
+// Host execution of the plan (double accumulation over its float data), for
+// tests of the plan construction against the factor's own solve.
+void triplan_solve_host(const TriPlanHost& P, const float* b, float* x);
+
+}  // namespace hdd
