@@ -192,6 +192,111 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
     }
 }
 
+// Heights whose supernodes all have <= kSliceRows (128) rows: one workgroup per
+// supernode does the whole height's work for it in one launch.
+// forward: t = b - (incidence sums) -> y = Linv t (LDS) -> c_g = L21_g y
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_fwd_small(int q0, const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                       const int* __restrict__ sz, const int* __restrict__ nr,
+                                                       const long long* __restrict__ doff, const float* __restrict__ diag,
+                                                       const int* __restrict__ inc_ptr, const long long* __restrict__ cpos,
+                                                       const int* __restrict__ orig_of, const float* __restrict__ b,
+                                                       const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                       const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y) {
+    __shared__ float ts[TriPlanHost::kSliceRows], ys[TriPlanHost::kSliceRows];
+    const int g = sn_list[q0 + blockIdx.x], n = sz[g], base = s0[g], m = nr[g];
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const int i = base + k;
+        float sum = 0.f;
+        for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
+        ts[k] = b[orig_of[i]] - sum;
+    }
+    __syncthreads();
+    const float* D = diag + doff[g];
+    const int lane = threadIdx.x & (G - 1);
+    constexpr int kU = TriPlanHost::kSliceRows / G;  // fixed trip count: every load of a row is issued at once
+    for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
+        const float* Dr = D + (long long)r * n;
+        float acc = 0.f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = lane + u * G;
+            if (k <= r) acc += Dr[k] * ts[k];
+        }
+        acc = group_sum<G>(acc);
+        if (lane == 0) { ys[r] = acc; y[base + r] = acc; }
+    }
+    __syncthreads();
+    const float* L = l21 + loff[g];
+    float* cg = c + cofs[g];
+    for (int a = (int)threadIdx.x / G; a < m; a += 256 / G) {
+        const float* La = L + (long long)a * n;
+        float acc = 0.f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int j = lane + u * G;
+            if (j < n) acc += La[j] * ys[j];
+        }
+        acc = group_sum<G>(acc);
+        if (lane == 0) cg[a] = acc;
+    }
+}
+
+// backward: t_j = y_j - sum_a L21_g[a, j] z[rows_g[a]] (four wavefronts split
+// the rows, lanes own columns j and j + 64, LDS reduction) -> z = Linv^T t
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_bwd_small(int q0, const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                       const int* __restrict__ sz, const int* __restrict__ nr,
+                                                       const int* __restrict__ roff, const int* __restrict__ srows,
+                                                       const long long* __restrict__ loff, const float* __restrict__ l21,
+                                                       const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                       const int* __restrict__ orig_of, const float* __restrict__ y,
+                                                       float* __restrict__ z, float* __restrict__ x) {
+    constexpr int kChunk = 1024;
+    __shared__ float zr[kChunk];
+    __shared__ float part[4][TriPlanHost::kSliceRows];
+    __shared__ float ts[TriPlanHost::kSliceRows];
+    const int g = sn_list[q0 + blockIdx.x], n = sz[g], base = s0[g], m = nr[g];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int* R = srows + roff[g];
+    const float* L = l21 + loff[g];
+    const int j0 = min(lane, n - 1), j1 = min(lane + 64, n - 1);
+    float acc0 = 0.f, acc1 = 0.f;
+    for (int a0 = 0; a0 < m; a0 += kChunk) {
+        const int cm = min(kChunk, m - a0);
+        for (int q = threadIdx.x; q < cm; q += 256) zr[q] = z[R[a0 + q]];
+        __syncthreads();
+#pragma unroll 4
+        for (int q = w; q < cm; q += 4) {
+            const float* La = L + (long long)(a0 + q) * n;
+            const float zq = zr[q];
+            acc0 += La[j0] * zq;
+            acc1 += La[j1] * zq;
+        }
+        __syncthreads();
+    }
+    if (lane < n) part[w][lane] = acc0;
+    if (lane + 64 < n) part[w][lane + 64] = acc1;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += 256)
+        ts[k] = y[base + k] - ((part[0][k] + part[1][k]) + (part[2][k] + part[3][k]));
+    __syncthreads();
+    const float* DT = diagT + doff[g];
+    const int gl = threadIdx.x & (G - 1);
+    constexpr int kU = TriPlanHost::kSliceRows / G;
+    for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
+        const float* Dr = DT + (long long)r * n;
+        float acc = 0.f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = r + gl + u * G;
+            if (k < n) acc += Dr[k] * ts[k];
+        }
+        acc = group_sum<G>(acc);
+        if (gl == 0) { z[base + r] = acc; x[orig_of[base + r]] = acc; }
+    }
+}
+
 // y = A x for CSR A (one thread per row, column order)
 __global__ void k_csr_mv(int rows, const int* __restrict__ p, const int* __restrict__ ix, const float* __restrict__ v,
                          const float* __restrict__ x, float* __restrict__ y) {
@@ -297,10 +402,10 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
     build_triplan(F, offsets, P);
     n = P.n; H = P.H; factor_nnz = P.factor_nnz;
     if (n == 0) return;
-    sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
-    max_sz = P.max_sz; big = P.big; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
+    sn_begin = P.sn_begin; sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
+    max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
     orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
-    sn_of.upload(P.sn_of); sl_sn.upload(P.sl_sn); sl_r0.upload(P.sl_r0); rowlist.upload(P.rowlist); inc_ptr.upload(P.inc_ptr);
+    sn_of.upload(P.sn_of); sn_list.upload(P.sn_list); sl_sn.upload(P.sl_sn); sl_r0.upload(P.sl_r0); rowlist.upload(P.rowlist); inc_ptr.upload(P.inc_ptr);
     ci_sn.upload(P.ci_sn); ci_a0.upload(P.ci_a0); ci_a1.upload(P.ci_a1); bi_sn.upload(P.bi_sn); bi_c.upload(P.bi_c); bi_j0.upload(P.bi_j0);
     loff.upload(P.loff); doff.upload(P.doff); cofs.upload(P.cofs); cpos.upload(P.cpos); bofs.upload(P.bofs);
     diag.upload(P.diag); diagT.upload(P.diagT); l21.upload(P.l21);
@@ -309,7 +414,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
 }
 
 size_t TriPlan::bytes() const {
-    return (orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
+    return (orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
             ci_a1.n + bi_sn.n + bi_c.n + bi_j0.n) * 4 +
            (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4;
 }
@@ -337,6 +442,14 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     float* yf = y.p;
     float* zb = y.p + n;
     for (int h = 0; h <= H; ++h) {
+        if (fused[h]) {
+            const int q0 = sn_begin[h], nq = sn_begin[h + 1] - sn_begin[h];
+            if (nq)
+                HDD_SN_DISPATCH(G[h], k_tri_fwd_small, nq, 0, s, q0, (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p,
+                                (const long long*)doff.p, (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p,
+                                (const int*)orig_of.p, b, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf);
+            continue;
+        }
         const int nrows = row_begin[h + 1] - row_begin[h];
         const int* rows = rowlist.p + row_begin[h];
         if (big[h] && nrows)
@@ -355,6 +468,14 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
                             (const float*)yf, c.p);
     }
     for (int h = H; h >= 0; --h) {
+        if (fused[h]) {
+            const int q0 = sn_begin[h], nq = sn_begin[h + 1] - sn_begin[h];
+            if (nq)
+                HDD_SN_DISPATCH(G[h], k_tri_bwd_small, nq, 0, s, q0, (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p,
+                                (const int*)roff.p, (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
+                                (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x);
+            continue;
+        }
         const int i0 = bi_begin[h], ni = bi_begin[h + 1] - bi_begin[h];
         if (ni)
             hipLaunchKernelGGL(k_tri_bwd_tiles, dim3(blocks_for(ni, 4)), dim3(256), 0, s, i0, i0 + ni, (const int*)bi_sn.p, (const int*)bi_c.p,

@@ -88,7 +88,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     const int nh = P.H + 1;
     P.sn_begin.assign(nh + 1, 0); P.sl_begin.assign(nh + 1, 0); P.row_begin.assign(nh + 1, 0);
     P.ci_begin.assign(nh + 1, 0); P.bi_begin.assign(nh + 1, 0);
-    P.max_sz.assign(nh, 0); P.big.assign(nh, 0); P.G.assign(nh, 4); P.Gc.assign(nh, 4); P.Gf.assign(nh, 4); P.Gb.assign(nh, 4);
+    P.max_sz.assign(nh, 0); P.big.assign(nh, 0); P.fused.assign(nh, 0); P.G.assign(nh, 4); P.Gc.assign(nh, 4); P.Gf.assign(nh, 4); P.Gb.assign(nh, 4);
     P.sl_rows.assign(nh, TriPlanHost::kSliceRows);
     for (int h = 0; h < nh; ++h) {
         int mx = 0;
@@ -122,6 +122,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
             chunks += (double)nch * n;
         }
         P.big[h] = P.max_sz[h] > TriPlanHost::kSliceRows;
+        P.fused[h] = !P.big[h] && (int)P.sn_list.size() - P.sn_begin[h] >= TriPlanHost::kFusedMinSupernodes;
         if (rows > 0) { P.G[h] = lanes_for(diag_len / rows); P.Gf[h] = lanes_for(incs / rows); P.Gb[h] = lanes_for(chunks / rows); }
         if (contrib_rows > 0) P.Gc[h] = lanes_for(contrib_len / contrib_rows);
     }

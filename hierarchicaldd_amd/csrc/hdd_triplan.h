@@ -37,7 +37,9 @@ struct TriPlanHost {
     std::vector<int> sn_of;                   // plan row -> supernode
     std::vector<long long> loff, doff;        // offsets into l21 / diag
     std::vector<float> diag, diagT, l21;
+    static constexpr int kFusedMinSupernodes = 1024;  // a height is done by one workgroup per supernode only with this many
     std::vector<int> sn_begin, max_sz, big;   // per height: supernode range in sn_list, largest supernode, 1 if > kSliceRows
+    std::vector<int> fused;                   // per height: 1 if every supernode is <= kSliceRows and there are many
     std::vector<int> sn_list;
     std::vector<int> sl_sn, sl_r0, sl_begin;  // row slices (supernode, first row) by height
     std::vector<int> sl_rows;                 // rows per slice, per height
