@@ -3,6 +3,9 @@
 #include "hdd_runtime.h"
 #include "hdd_sigma_device.h"
 #include "hdd_spectral.h"
+#include "hdd_triplan.h"
+#include "hdd_parallel.h"
+#include "hdd_factor.h"
 #include <cstring>
 #include <string>
 
@@ -267,6 +270,49 @@ hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
                     G ? G->bottom_seconds : 0.0, G ? (double)G->bottom_steps : 0.0, G ? G->bottom_check : -1.0};
     for (int i = 0; i < n && i < 13; ++i) out[i] = v[i];
     return HDD_OK;
+}
+
+static void plan_stats_impl(const SigmaHost& H, int level, int which, double* out, int cap, int* heights) {
+    if (level < 0 || level >= H.levels) throw std::runtime_error("no such sigma level");
+    std::vector<SupernodalFactor> f;
+    std::vector<int> offs;
+    if (which == 0) {
+        f.resize(H.nsub);
+        ParallelErrors errs;
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < H.nsub; ++s)
+            if (H.Aii[level][s].rows > 0) errs.run([&] { factor_spd(H.Aii[level][s], H.Aii_coords[level][s], 1e-5, f[s]); });
+        errs.rethrow();
+        int off = 0;
+        for (int s = 0; s < H.nsub; ++s) { offs.push_back(off); off += H.Aii[level][s].rows; }
+    } else {
+        f.resize(1);
+        std::vector<Int3> rc(H.if_coords[level].size());
+        for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[level][i].x << level, H.if_coords[level][i].y << level, H.if_coords[level][i].z << level};
+        factor_spd(H.Arr[level], rc, 1e-5, f[0]);
+        offs.push_back(0);
+    }
+    std::vector<const SupernodalFactor*> fp;
+    std::vector<int> fo;
+    for (size_t s = 0; s < f.size(); ++s) if (f[s].n) { fp.push_back(&f[s]); fo.push_back(offs[s]); }
+    TriPlanHost P;
+    build_triplan(fp, fo, P);
+    *heights = P.H + 1;
+    std::vector<double> v(5 * (size_t)(P.H + 1), 0.0);
+    for (int h = 0; h <= P.H; ++h) {
+        for (int q = P.sn_begin[h]; q < P.sn_begin[h + 1]; ++q) {
+            const int g = P.sn_list[q];
+            v[5 * h + 0] += 1; v[5 * h + 1] += P.sz[g];
+            v[5 * h + 2] += 0.5 * P.sz[g] * (P.sz[g] + 1.0); v[5 * h + 3] += (double)P.sz[g] * P.nr[g];
+        }
+        v[5 * h + 4] = P.max_sz[h];
+    }
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+}
+
+hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out, int cap, int* heights) {
+    if (!ctx || !heights || cap < 0 || (cap > 0 && !out) || (which != 0 && which != 1)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, plan_stats_impl(ctx->solver->sigma_host, level, which, out, cap, heights));
 }
 
 hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max) {

@@ -35,6 +35,7 @@ struct TriPlan {
 
 struct SigmaLevel {
     int n_if = 0, n_int = 0;
+    int n_distinct = 0;  // distinct interior matrices (subdomains of the same shape share one factor)
     CsrDev Air, AirT, Arr, P, PT;  // P: this level -> next coarser (rows = this level), PT its transpose
     TriPlan Lii, Lrr;
     DBuf<float> u, b, r, vb, vx, tmp;

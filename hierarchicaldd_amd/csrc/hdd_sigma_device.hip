@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <string>
+#include <map>
 #include <omp.h>
 #include "hdd_factor.h"
 #include "hdd_parallel.h"
@@ -497,6 +498,36 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
 
 SigmaDevice::~SigmaDevice() {}
 
+// For each matrix, the index of the first matrix equal to it (same rows,
+// pattern and values), or its own index.
+static std::vector<int> identical_matrices(const std::vector<Csr>& A) {
+    auto hash = [](const Csr& m) {
+        uint64_t h = 1469598103934665603ULL;
+        auto mix = [&](const void* p, size_t bytes) {
+            const unsigned char* c = static_cast<const unsigned char*>(p);
+            for (size_t i = 0; i < bytes; ++i) { h ^= c[i]; h *= 1099511628211ULL; }
+        };
+        mix(&m.rows, sizeof(m.rows));
+        mix(m.ptr.data(), m.ptr.size() * sizeof(int));
+        mix(m.idx.data(), m.idx.size() * sizeof(int));
+        mix(m.val.data(), m.val.size() * sizeof(float));
+        return h;
+    };
+    const int n = (int)A.size();
+    std::vector<uint64_t> h(n);
+#pragma omp parallel for schedule(dynamic)
+    for (int s = 0; s < n; ++s) h[s] = hash(A[s]);
+    std::vector<int> rep(n);
+    std::map<uint64_t, std::vector<int>> seen;
+    for (int s = 0; s < n; ++s) {
+        rep[s] = s;
+        for (int c : seen[h[s]])
+            if (A[c].rows == A[s].rows && A[c].ptr == A[s].ptr && A[c].idx == A[s].idx && A[c].val == A[s].val) { rep[s] = c; break; }
+        if (rep[s] == s) seen[h[s]].push_back(s);
+    }
+    return rep;
+}
+
 // 200 sweeps leave l^200 of an eigenmode with eigenvalue l; modes with l^200 < 1e-9 are dropped
 static const double kBottomLambdaCut = std::pow(1e-9, 1.0 / 200.0);
 static const size_t kBottomBasisBytes = size_t(2) << 30;
@@ -564,16 +595,20 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing
         L.AirT.upload(transpose(air));
         L.Arr.upload(H.Arr[l]);
         if (l + 1 < H.levels) { L.P.upload(H.Prr[l]); L.PT.upload(transpose(H.Prr[l])); }
+        // subdomains of the same shape have identical Aii: factor each distinct matrix once
+        const std::vector<int> rep = identical_matrices(H.Aii[l]);
         std::vector<SupernodalFactor> fi(H.nsub);
         ParallelErrors errs;
 #pragma omp parallel for schedule(dynamic)
         for (int s = 0; s < H.nsub; ++s)
-            if (H.Aii[l][s].rows > 0) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, fi[s]); });
+            if (H.Aii[l][s].rows > 0 && rep[s] == s) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, fi[s]); });
         errs.rethrow();
         std::vector<const SupernodalFactor*> fp;
         std::vector<int> fo;
-        for (int s = 0; s < H.nsub; ++s) if (H.Aii[l][s].rows > 0) { fp.push_back(&fi[s]); fo.push_back(off[s]); }
+        for (int s = 0; s < H.nsub; ++s) if (H.Aii[l][s].rows > 0) { fp.push_back(&fi[rep[s]]); fo.push_back(off[s]); }
         L.Lii.build(fp, fo);
+        L.n_distinct = 0;
+        for (int s = 0; s < H.nsub; ++s) L.n_distinct += H.Aii[l][s].rows > 0 && rep[s] == s;
         std::vector<Int3> rc(H.if_coords[l].size());
         for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l};
         SupernodalFactor fr;

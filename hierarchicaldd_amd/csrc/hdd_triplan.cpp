@@ -1,5 +1,6 @@
 #include "hdd_triplan.h"
 #include <algorithm>
+#include <map>
 #include <stdexcept>
 #include <omp.h>
 
@@ -27,10 +28,15 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     std::vector<int> height;
     struct Ref { int m, s; };
     std::vector<Ref> refs;
+    std::vector<int> owner;  // per supernode: 1 if it holds the data (first member of its factor)
+    std::map<const SupernodalFactor*, int> first_gid;  // factor -> global id of its first supernode, first member
     long long dsize = 0, lsize = 0;
     for (size_t m = 0; m < F.size(); ++m) {
         const SupernodalFactor& f = *F[m];
         const int off = offsets[m];
+        const auto seen = first_gid.find(&f);
+        const int shared = seen == first_gid.end() ? -1 : seen->second;
+        if (shared < 0) first_gid[&f] = (int)P.s0.size();
         for (int k = 0; k < f.n; ++k) P.orig_of[off + k] = off + f.perm[k];
         for (size_t s = 0; s < f.sn.size(); ++s) {
             const Supernode& S = f.sn[s];
@@ -40,8 +46,14 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
             P.roff.push_back((int)P.srows.size());
             for (int r : S.rows) P.srows.push_back(off + r);
             for (int k = 0; k < S.n; ++k) P.sn_of[off + S.s0 + k] = g;
-            P.doff.push_back(dsize); dsize += (long long)S.n * S.n;
-            P.loff.push_back(lsize); lsize += (long long)S.n * S.rows.size();
+            if (shared < 0) {
+                P.doff.push_back(dsize); dsize += (long long)S.n * S.n;
+                P.loff.push_back(lsize); lsize += (long long)S.n * S.rows.size();
+            } else {  // a repeated factor (identical matrix): its blocks are stored once
+                P.doff.push_back(P.doff[shared + s]);
+                P.loff.push_back(P.loff[shared + s]);
+            }
+            owner.push_back(shared < 0);
             height.push_back(S.height);
             refs.push_back({(int)m, (int)s});
         }
@@ -50,6 +62,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     P.diag.assign(dsize, 0.f); P.diagT.assign(dsize, 0.f); P.l21.assign(lsize, 0.f);
 #pragma omp parallel for schedule(dynamic)
     for (int g = 0; g < nsn; ++g) {
+        if (!owner[g]) continue;
         const SupernodalFactor& f = *F[refs[g].m];
         const int s = refs[g].s, ns = P.sz[g], nr = P.nr[g];
         std::vector<double> inv(f.L11[s]);
@@ -102,8 +115,11 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         P.ci_begin[h] = (int)P.ci_sn.size();
         P.bi_begin[h] = (int)P.bi_sn.size();
         double rows = 0, diag_len = 0, contrib_rows = 0, contrib_len = 0, incs = 0, chunks = 0;
-        for (int g = 0; g < nsn; ++g) {
-            if (height[g] != h) continue;
+        std::vector<int> at_h;
+        for (int g = 0; g < nsn; ++g) if (height[g] == h) at_h.push_back(g);
+        // supernodes that share blocks run next to each other (the shared blocks stay in cache)
+        std::stable_sort(at_h.begin(), at_h.end(), [&](int a, int b) { return P.loff[a] != P.loff[b] ? P.loff[a] < P.loff[b] : P.doff[a] < P.doff[b]; });
+        for (int g : at_h) {
             const int n = P.sz[g], nr = P.nr[g];
             P.sn_list.push_back(g);
             P.max_sz[h] = std::max(P.max_sz[h], n);

@@ -19,6 +19,11 @@
 //   reduce  t_j = y_j - sum_c p_g[c, j]                                   (heights with big supernodes)
 //   diag    z_g = Linv_g^T t_g
 // Every sum runs in a fixed order, so results are deterministic.
+//
+// A factor passed more than once (identical matrices: subdomains of the same
+// shape) keeps one copy of its blocks; its members get their own rows and
+// partial sums, and within a height the supernodes that share blocks are
+// scheduled next to each other, so the shared blocks are read from cache.
 #pragma once
 #include <vector>
 #include "hdd_factor.h"

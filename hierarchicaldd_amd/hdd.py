@@ -42,7 +42,7 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
            "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats",
-           "hdd_bottom_spectrum", "hdd_profile_copy"]
+           "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats"]
 
 
 def lib():
@@ -77,6 +77,7 @@ def lib():
         L.hdd_setup_seconds.argtypes = [P, C.POINTER(C.c_double)]
         L.hdd_profile_sweep.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hdd_profile_copy.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hdd_sigma_plan_stats.argtypes = [P, I, I, P, I, C.POINTER(I)]
         L.hdd_stats.argtypes = [P, P, I]
         L.hdd_bottom_spectrum.argtypes = [P, C.c_double, P, I, C.POINTER(I), C.POINTER(C.c_double)]
         for name in EXPORTS:
@@ -235,6 +236,14 @@ class DDSolver:
         out = np.zeros(len(keys), dtype=np.float64)
         self._check(lib().hdd_stats(self.h, _ptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
+
+    def plan_stats(self, level: int, which: str):
+        """Per tree height of a triangular-solve plan ("Aii" batch or "Arr"): supernodes, rows,
+        packed diagonal entries, L21 entries, largest supernode."""
+        out = np.zeros(5 * 64, dtype=np.float64)
+        h = C.c_int()
+        self._check(lib().hdd_sigma_plan_stats(self.h, level, {"Aii": 0, "Arr": 1}[which], _ptr(out), out.size, C.byref(h)))
+        return out[:5 * min(h.value, 64)].reshape(-1, 5)
 
     def bottom_spectrum(self, lambda_cut: float, cap: int = 1024):
         """Eigenvalues >= lambda_cut of the bottom sigma smoother's iteration matrix (host, double)."""

@@ -164,6 +164,15 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * n = number of slots provided (up to 13). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
+/* Shape of the triangular-solve plan of sigma level `level`: which = 0 for
+ * the batch of all subdomains' (Aii + eps I), 1 for (Arr + eps I). Factors on
+ * the host (works on host-only contexts) and writes 5 values per tree height
+ * h into out[5h .. 5h+4]: supernodes, rows, packed diagonal-inverse entries,
+ * sub-diagonal (L21) entries, largest supernode; *heights = tree heights (only
+ * the first cap/5 are written). Used to state the algorithmic bytes of the
+ * solve kernels. No reference counterpart. */
+hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out, int cap, int* heights);
+
 /* Eigenvalues >= lambda_cut (0 < lambda_cut < 1) of the bottom sigma
  * smoother's iteration matrix (Arr+eI)^-1 sum_s Air_s^T (Aii_s+eI)^-1 Air_s,
  * descending, computed on the host in double precision (works on host-only
