@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 METRIC = "256³ time-to-1e-6 rel-res; batched solves/s; p50/p99 served latency"
 
 
@@ -109,10 +110,43 @@ def main():
                    "tolerance": "max|r| <= 1e-6 max|b|", "iterations": int(np.median(iters)),
                    "time_to_tolerance_s": float(np.mean(times)), "setup_s": st["setup_s"]},
         "roofline": {"kernel": "k_smooth (level-0 damped-Jacobi sweep)", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": counter_traffic(dom.n_pages) if args.config == "sphere256" else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def counter_traffic(pages):
+    """HBM bytes of one level-0 sweep from the committed rocprofv3 counter summaries.
+
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the sweep kernel, each scaled by
+    what the same counter reads for the calibration copy kernel of the same runs,
+    which reads and writes (pages + 1) * 256 floats per launch (hdd_profile_copy).
+    None when the summaries are absent or were taken on another page count.
+    """
+    def read(name, counter):
+        path = os.path.join(PROFILES, name)
+        if not os.path.exists(path):
+            return None
+        vals = {}
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 5 and f[-3] == counter and f[-4].isdigit():
+                vals.setdefault(f[0].split("::")[-1], {})[int(f[-4])] = float(f[-1])
+        return vals
+    fetch = read("k_smooth_sphere256_fetch_size.txt", "FETCH_SIZE")
+    write = read("k_smooth_sphere256_write_size.txt", "WRITE_SIZE")
+    if not fetch or not write or "k_smooth" not in fetch or "k_copy" not in fetch:
+        return None
+    copy_kib = (pages + 1) * 256 * 4 / 1024.0
+    copy_grid = max(fetch["k_copy"], key=lambda g: fetch["k_copy"][g])
+    f_scale = copy_kib / fetch["k_copy"][copy_grid]
+    w_scale = copy_kib / write["k_copy"][copy_grid]
+    sweep_grid = max(fetch["k_smooth"], key=lambda g: g)
+    if sweep_grid < pages * 256:
+        return None
+    return 1024.0 * (fetch["k_smooth"][sweep_grid] * f_scale + write["k_smooth"][sweep_grid] * w_scale)
 
 
 def cpu_baseline(hdd):

@@ -25,6 +25,17 @@ __constant__ float c_omega_fc[8];   // (float)(2/3) / (float)fc, fc = 0..6
 __constant__ float c_rweights[64];  // restriction weights, [i][j][k] with w = {1,3,3,1}/8
 __constant__ float c_pweights[8];   // prolongation weights 4 * {3/4,1/4}^3, [i][j][k]
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i mod 8), and each XCD has its own L2. Giving XCD x the contiguous page range
+// [x*chunk, (x+1)*chunk) keeps Morton neighbours, whose halo cells a stencil
+// re-reads, in the same L2. Launch with kXcds * chunk workgroups; ids past np
+// return.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_chunk(int np) { return (np + kXcds - 1) / kXcds; }
+__device__ __forceinline__ int xcd_page(int np) {
+    return (int)(blockIdx.x % kXcds) * xcd_chunk(np) + (int)(blockIdx.x / kXcds);
+}
+
 __device__ __forceinline__ int nb_index(const int* __restrict__ nbr, int p, int c, int f) {
     switch (f) {
         case 0: return (c >= 64) ? p * 256 + c - 64 : nbr[p * 6 + 0] * 256 + c + 192;
@@ -57,7 +68,8 @@ __device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int 
 // Cells outside the selection are copied unchanged.
 __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restrict__ u_in, float* __restrict__ u_out,
                                                 const float* __restrict__ b, int write_interior, int write_boundary) {
-    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
+    if (p >= L.np) return;
     const uint32_t f = L.flags[e];
     const uint32_t sel = f & (kActive | kBoundary);
     const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
@@ -69,7 +81,8 @@ __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restr
 
 __global__ void __launch_bounds__(256) k_residual(DevLevel L, const float* __restrict__ u, const float* __restrict__ b,
                                                   float* __restrict__ r) {
-    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
+    if (p >= L.np) return;
     const uint32_t f = L.flags[e];
     if (f & kActive) r[e] = cell_residual(L.nbr, p, c, f, u, b[e]);
 }
@@ -92,7 +105,8 @@ __device__ __forceinline__ int lut_page(const DevLevel& L, int x, int y, int z) 
 
 // coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only)
 __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const float* __restrict__ r_f, float* __restrict__ b_c) {
-    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const int p = xcd_page(C.np), c = threadIdx.x, e = p * 256 + c;
+    if (p >= C.np) return;
     if (!(C.flags[e] & kActive)) return;
     const int x = C.base[p * 3 + 0] + (c >> 6), y = C.base[p * 3 + 1] + ((c >> 3) & 7), z = C.base[p * 3 + 2] + (c & 7);
     const int fx = 2 * x - 2, fy = 2 * y - 2, fz = 2 * z - 2;
@@ -109,7 +123,8 @@ __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const 
 
 // fine u += sum_{2x2x2} w u_coarse (Active fine cells only)
 __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, float* __restrict__ u_f, const float* __restrict__ u_c) {
-    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const int p = xcd_page(F.np), c = threadIdx.x, e = p * 256 + c;
+    if (p >= F.np) return;
     if (!(F.flags[e] & kActive)) return;
     const int x = F.base[p * 3 + 0] + (c >> 6), y = F.base[p * 3 + 1] + ((c >> 3) & 7), z = F.base[p * 3 + 2] + (c & 7);
     const int cx = (x + 1) >> 1, cy = (y + 1) >> 1, cz = (z + 1) >> 1;
@@ -127,7 +142,8 @@ __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, float* 
 
 // PCG operator: out = L v on Active|Interface cells (double accumulation), 0 elsewhere (fused Project)
 __global__ void __launch_bounds__(256) k_laplace_pcg(DevLevel L, const float* __restrict__ v, float* __restrict__ out) {
-    const int p = blockIdx.x, c = threadIdx.x, e = p * 256 + c;
+    const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
+    if (p >= L.np) return;
     const uint32_t f = L.flags[e];
     float res = 0.f;
     if (f & (kActive | kInterface)) {
@@ -253,10 +269,10 @@ void upload_constants() {
 }
 
 void launch_smooth(const DevLevel& L, const float* u_in, float* u_out, const float* b, bool interior, bool boundary, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_smooth, dim3(L.np), dim3(256), 0, s, L, u_in, u_out, b, (int)interior, (int)boundary);
+    if (L.np) hipLaunchKernelGGL(k_smooth, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u_in, u_out, b, (int)interior, (int)boundary);
 }
 void launch_residual(const DevLevel& L, const float* u, const float* b, float* r, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_residual, dim3(L.np), dim3(256), 0, s, L, u, b, r);
+    if (L.np) hipLaunchKernelGGL(k_residual, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u, b, r);
 }
 void launch_fill(float* a, float v, size_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, a, v, n);
@@ -265,13 +281,13 @@ void launch_copy(float* d, const float* src, size_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, d, src, n);
 }
 void launch_restrict(const DevLevel& F, const DevLevel& C, const float* r_f, float* b_c, hipStream_t s) {
-    if (C.np) hipLaunchKernelGGL(k_restrict, dim3(C.np), dim3(256), 0, s, F, C, r_f, b_c);
+    if (C.np) hipLaunchKernelGGL(k_restrict, dim3(kXcds * xcd_chunk(C.np)), dim3(256), 0, s, F, C, r_f, b_c);
 }
 void launch_prolong(const DevLevel& F, const DevLevel& C, float* u_f, const float* u_c, hipStream_t s) {
-    if (F.np) hipLaunchKernelGGL(k_prolong, dim3(F.np), dim3(256), 0, s, F, C, u_f, u_c);
+    if (F.np) hipLaunchKernelGGL(k_prolong, dim3(kXcds * xcd_chunk(F.np)), dim3(256), 0, s, F, C, u_f, u_c);
 }
 void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_laplace_pcg, dim3(L.np), dim3(256), 0, s, L, v, out);
+    if (L.np) hipLaunchKernelGGL(k_laplace_pcg, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, v, out);
 }
 void launch_project(const DevLevel& L, float* v, hipStream_t s) {
     if (L.np) hipLaunchKernelGGL(k_project, dim3(L.np), dim3(256), 0, s, L, v);
