@@ -272,6 +272,35 @@ hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
     return HDD_OK;
 }
 
+hdd_status hdd_profile_solve(hdd_ctx* ctx, int level, int which, int reps, double* avg_ms, double* bytes) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx || reps <= 0 || !avg_ms || which < 0 || which > 2) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        Solver& S = *ctx->solver;
+        if (!S.sigma || level < 0 || level >= S.sigma->levels) throw std::runtime_error("no such sigma level");
+        SigmaLevel& L = S.sigma->lv[level];
+        TriPlan* P = which == 0 ? &L.Lii : which == 1 ? &L.Lrr : &S.sigma->Lbottom;
+        if (which == 2 && (level != S.sigma->levels - 1 || !S.sigma->bottom_direct)) throw std::runtime_error("no combined plan at this level");
+        if (P->n == 0) throw std::runtime_error("empty plan");
+        DBuf<float> bv, xv;  // zero right-hand side: the solve's cost does not depend on the values
+        bv.alloc(P->n); xv.alloc(P->n);
+        const float* b = bv.p;
+        float* x = xv.p;
+        hipEvent_t a, e;
+        hdd_check(hipEventCreate(&a)); hdd_check(hipEventCreate(&e));
+        P->solve(b, x, S.stream);
+        hdd_check(hipEventRecord(a, S.stream));
+        for (int i = 0; i < reps; ++i) P->solve(b, x, S.stream);
+        hdd_check(hipEventRecord(e, S.stream));
+        hdd_check(hipEventSynchronize(e));
+        float ms = 0.f;
+        hdd_check(hipEventElapsedTime(&ms, a, e));
+        (void)hipEventDestroy(a); (void)hipEventDestroy(e);
+        *avg_ms = ms / reps;
+        if (bytes) *bytes = 8.0 * (double)P->factor_nnz;
+    })
+}
+
 static void plan_stats_impl(const SigmaHost& H, int level, int which, double* out, int cap, int* heights) {
     if (level < 0 || level >= H.levels) throw std::runtime_error("no such sigma level");
     std::vector<SupernodalFactor> f;

@@ -42,7 +42,8 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
            "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats",
-           "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats"]
+           "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats",
+           "hdd_profile_solve"]
 
 
 def lib():
@@ -78,6 +79,7 @@ def lib():
         L.hdd_profile_sweep.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hdd_profile_copy.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hdd_sigma_plan_stats.argtypes = [P, I, I, P, I, C.POINTER(I)]
+        L.hdd_profile_solve.argtypes = [P, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hdd_stats.argtypes = [P, P, I]
         L.hdd_bottom_spectrum.argtypes = [P, C.c_double, P, I, C.POINTER(I), C.POINTER(C.c_double)]
         for name in EXPORTS:
@@ -236,6 +238,12 @@ class DDSolver:
         out = np.zeros(len(keys), dtype=np.float64)
         self._check(lib().hdd_stats(self.h, _ptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))
+
+    def profile_solve(self, level: int, which: str, reps: int = 20):
+        """Mean duration (ms) of one triangular-solve plan ("Aii", "Arr" or "bottom") and its algorithmic bytes."""
+        ms, by = C.c_double(), C.c_double()
+        self._check(lib().hdd_profile_solve(self.h, level, {"Aii": 0, "Arr": 1, "bottom": 2}[which], reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def plan_stats(self, level: int, which: str):
         """Per tree height of a triangular-solve plan ("Aii" batch or "Arr"): supernodes, rows,

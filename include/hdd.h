@@ -164,6 +164,15 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * n = number of slots provided (up to 13). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
+/* Measurement: runs `reps` exact solves of one triangular-solve plan of sigma
+ * level `level` (which = 0: all subdomains' (Aii + eps I), 1: (Arr + eps I),
+ * 2: the bottom level's combined system) on the context stream, bracketed by
+ * HIP events; returns the mean duration of one solve and its algorithmic
+ * bytes (4 B per factor entry per pass, two passes, every subdomain counted).
+ * The right-hand side is zero (the cost does not depend on the values). No
+ * reference counterpart. */
+hdd_status hdd_profile_solve(hdd_ctx* ctx, int level, int which, int reps, double* avg_ms, double* bytes_per_solve);
+
 /* Shape of the triangular-solve plan of sigma level `level`: which = 0 for
  * the batch of all subdomains' (Aii + eps I), 1 for (Arr + eps I). Factors on
  * the host (works on host-only contexts) and writes 5 values per tree height
