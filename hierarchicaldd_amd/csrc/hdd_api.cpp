@@ -29,28 +29,32 @@ static thread_local std::string g_create_error;
 
 extern "C" {
 
-static hdd_status create_impl(const hdd_problem_desc* d, hdd_ctx** out, bool host_only) {
-    if (!d || !out || d->n_pages < 0 || (d->n_pages > 0 && (!d->page_offsets || !d->flags || !d->b))) {
-        g_create_error = "hdd_create: invalid arguments";
-        return HDD_ERR_ARG;
-    }
+static bool desc_ok(const hdd_problem_desc* d, std::string& why) {
+    if (!d || d->n_pages < 0 || (d->n_pages > 0 && (!d->page_offsets || !d->flags || !d->b))) { why = "invalid arguments"; return false; }
     for (int a = 0; a < 3; ++a)
-        if (d->size[a] < 2 || d->sub_size[a] < 2) { g_create_error = "hdd_create: size and sub_size must be >= 2"; return HDD_ERR_ARG; }
+        if (d->size[a] < 2 || d->sub_size[a] < 2) { why = "size and sub_size must be >= 2"; return false; }
     if (d->vlevels < 1 || d->sigma_levels < 1 || d->nVcycles < 0 || d->interior_smoothing < 0 || d->boundary_smoothing < 0 ||
-        d->sigma_smoothing < 0 || d->sigma_mg < 0) {
-        g_create_error = "hdd_create: invalid solver parameters";
-        return HDD_ERR_ARG;
-    }
+        d->sigma_smoothing < 0 || d->sigma_mg < 0) { why = "invalid solver parameters"; return false; }
+    return true;
+}
+
+static SolverParams params_of(const hdd_problem_desc* d) {
+    SolverParams p;
+    p.size = {d->size[0], d->size[1], d->size[2]};
+    p.sub = {d->sub_size[0], d->sub_size[1], d->sub_size[2]};
+    p.vlevels = d->vlevels; p.sigma_levels = d->sigma_levels; p.nV = d->nVcycles;
+    p.int_s = d->interior_smoothing; p.bnd_s = d->boundary_smoothing;
+    p.sig_s = d->sigma_smoothing; p.sig_mg = d->sigma_mg;
+    return p;
+}
+
+static hdd_status create_impl(const hdd_problem_desc* d, hdd_ctx** out, bool host_only) {
+    std::string why;
+    if (!out || !desc_ok(d, why)) { g_create_error = "hdd_create: " + (out ? why : std::string("invalid arguments")); return HDD_ERR_ARG; }
     try {
-        SolverParams p;
-        p.size = {d->size[0], d->size[1], d->size[2]};
-        p.sub = {d->sub_size[0], d->sub_size[1], d->sub_size[2]};
-        p.vlevels = d->vlevels; p.sigma_levels = d->sigma_levels; p.nV = d->nVcycles;
-        p.int_s = d->interior_smoothing; p.bnd_s = d->boundary_smoothing;
-        p.sig_s = d->sigma_smoothing; p.sig_mg = d->sigma_mg;
         auto* c = new hdd_ctx();
         try {
-            c->solver.reset(new Solver(p, d->n_pages, d->page_offsets, d->flags, d->b, host_only));
+            c->solver.reset(new Solver(params_of(d), d->n_pages, d->page_offsets, d->flags, d->b, host_only));
         } catch (...) { delete c; throw; }
         *out = c;
         return HDD_OK;
@@ -62,6 +66,81 @@ static hdd_status create_impl(const hdd_problem_desc* d, hdd_ctx** out, bool hos
 
 hdd_status hdd_create(const hdd_problem_desc* d, hdd_ctx** out) { return create_impl(d, out, false); }
 hdd_status hdd_create_host_only(const hdd_problem_desc* d, hdd_ctx** out) { return create_impl(d, out, true); }
+
+hdd_status hdd_create_batch(const hdd_problem_desc* descs, int count, hdd_ctx** out) {
+    if (!descs || !out || count < 1) { g_create_error = "hdd_create_batch: invalid arguments"; return HDD_ERR_ARG; }
+    for (int i = 0; i < count; ++i) {
+        std::string why;
+        if (!desc_ok(descs + i, why)) { g_create_error = "hdd_create_batch: problem " + std::to_string(i) + ": " + why; return HDD_ERR_ARG; }
+    }
+    try {
+        std::vector<std::shared_ptr<ProblemHost>> probs;
+        for (int i = 0; i < count; ++i) {
+            const hdd_problem_desc* d = descs + i;
+            probs.push_back(std::make_shared<ProblemHost>(params_of(d), d->n_pages, d->page_offsets, d->flags, d->b));
+        }
+        auto* c = new hdd_ctx();
+        try {
+            c->solver.reset(new Solver(probs));
+        } catch (...) { delete c; throw; }
+        *out = c;
+        return HDD_OK;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return HDD_ERR_SETUP;
+    }
+}
+
+hdd_status hdd_batch_count(const hdd_ctx* ctx, int* count) {
+    if (!ctx || !count) return HDD_ERR_ARG;
+    *count = ctx->solver->nprob();
+    return HDD_OK;
+}
+
+hdd_status hdd_batch_num_pages(const hdd_ctx* ctx, int problem, int* n) {
+    if (!ctx || !n || problem < 0 || problem >= ctx->solver->nprob()) return HDD_ERR_ARG;
+    *n = ctx->solver->prob[problem]->lv[0].npages();
+    return HDD_OK;
+}
+
+hdd_status hdd_batch_page_offsets(const hdd_ctx* ctx, int problem, uint64_t* out) {
+    if (!ctx || !out || problem < 0 || problem >= ctx->solver->nprob()) return HDD_ERR_ARG;
+    const PagedLevel& L = ctx->solver->prob[problem]->lv[0];
+    for (int p = 0; p < L.npages(); ++p) out[p] = page_key(L.page_base[p].x >> 2, L.page_base[p].y >> 3, L.page_base[p].z >> 3) << 12;
+    return HDD_OK;
+}
+
+hdd_status hdd_batch_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_report* rep, double* hist, int hist_len) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx || !prm || hist_len < 0) return HDD_ERR_ARG;
+    const int np = ctx->solver->nprob();
+    for (int p = 0; p < np; ++p)
+        if (prm[p].max_iterations < 0) return HDD_ERR_ARG;
+    try {
+        std::vector<double> tol(np);
+        std::vector<int> mit(np);
+        for (int p = 0; p < np; ++p) { tol[p] = prm[p].tolerance; mit[p] = prm[p].max_iterations; }
+        std::vector<PcgResult> res;
+        ctx->solver->pcg_batch(tol, mit, res);
+        bool all = true;
+        for (int p = 0; p < np; ++p) {
+            const PcgResult& r = res[p];
+            if (rep) { rep[p].iterations = r.iterations; rep[p].seconds = r.seconds; rep[p].final_norm = r.hist.empty() ? 0.0 : r.hist.back(); }
+            if (hist) for (int i = 0; i < hist_len; ++i) hist[(size_t)p * hist_len + i] = i < (int)r.hist.size() ? r.hist[i] : 0.0;
+            all = all && r.iterations >= 0;
+        }
+        return all ? HDD_OK : HDD_ERR_NOT_CONVERGED;
+    } catch (std::exception& e) {
+        ctx->err = e.what();
+        return HDD_ERR_DEVICE;
+    }
+}
+
+hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx || !host || problem < 0 || problem >= ctx->solver->nprob()) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->download_x(problem, host));
+}
 
 hdd_status hdd_destroy(hdd_ctx* ctx) {
     if (!ctx) return HDD_ERR_ARG;
@@ -146,27 +225,27 @@ hdd_status hdd_apply_preconditioner(hdd_ctx* ctx) {
 
 hdd_status hdd_sigma_info(const hdd_ctx* ctx, int* levels, int* nsub) {
     if (!ctx) return HDD_ERR_ARG;
-    if (levels) *levels = ctx->solver->sigma_host.levels;
-    if (nsub) *nsub = ctx->solver->sigma_host.nsub;
+    if (levels) *levels = ctx->solver->sigma_host().levels;
+    if (nsub) *nsub = ctx->solver->sigma_host().nsub;
     return HDD_OK;
 }
 
 hdd_status hdd_interface_dofs(const hdd_ctx* ctx, int level, int* n) {
-    if (!ctx || !n || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
-    *n = ctx->solver->sigma_host.n_if[level];
+    if (!ctx || !n || level < 0 || level >= ctx->solver->sigma_host().levels) return HDD_ERR_ARG;
+    *n = ctx->solver->sigma_host().n_if[level];
     return HDD_OK;
 }
 
 hdd_status hdd_interface_coords(const hdd_ctx* ctx, int level, int* xyz) {
-    if (!ctx || !xyz || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
-    const auto& v = ctx->solver->sigma_host.if_coords[level];
+    if (!ctx || !xyz || level < 0 || level >= ctx->solver->sigma_host().levels) return HDD_ERR_ARG;
+    const auto& v = ctx->solver->sigma_host().if_coords[level];
     for (size_t i = 0; i < v.size(); ++i) { xyz[3 * i] = v[i].x; xyz[3 * i + 1] = v[i].y; xyz[3 * i + 2] = v[i].z; }
     return HDD_OK;
 }
 
 hdd_status hdd_interface_matrix(const hdd_ctx* ctx, int which, int level, int sd, int* rows, int* cols, int* nnz, int* ptr, int* idx, float* val) {
     if (!ctx) return HDD_ERR_ARG;
-    const SigmaHost& S = ctx->solver->sigma_host;
+    const SigmaHost& S = ctx->solver->sigma_host();
     if (level < 0 || level >= S.levels || which < 0 || which > 3) return HDD_ERR_ARG;
     if ((which == 1 || which == 2) && (sd < 0 || sd >= S.nsub)) return HDD_ERR_ARG;
     if (which == 3 && level >= S.levels - 1) return HDD_ERR_ARG;
@@ -188,7 +267,7 @@ hdd_status hdd_sigma_vcycle(hdd_ctx* ctx, const float* b, float* u) {
 
 hdd_status hdd_sigma_smooth(hdd_ctx* ctx, int level, const float* b, float* u) {
     if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
-    if (!ctx || !b || !u || level < 0 || level >= ctx->solver->sigma_host.levels) return HDD_ERR_ARG;
+    if (!ctx || !b || !u || level < 0 || level >= ctx->solver->sigma_host().levels) return HDD_ERR_ARG;
     HDD_TRY(ctx, { sigma_smooth_host(*ctx->solver->sigma, level, b, u, ctx->solver->stream); })
 }
 
@@ -263,9 +342,11 @@ hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
     if (!ctx || !out) return HDD_ERR_ARG;
     const Solver& S = *ctx->solver;
     const SigmaDevice* G = S.sigma.get();
+    int nsub = 0;
+    for (auto& q : S.prob) nsub += q->sigma.nsub;
     double v[13] = {S.setup_seconds, G ? G->factor_seconds : 0.0, G ? (double)G->plan_bytes : 0.0,
-                    (double)S.lv[0].host.npages(), S.sigma_host.levels ? (double)S.sigma_host.n_if[0] : 0.0,
-                    (G && G->levels) ? (double)G->lv[0].n_int : 0.0, (double)S.sigma_host.nsub,
+                    (double)S.lv[0].host.npages(), (G && G->levels) ? (double)G->lv[0].n_if : (S.sigma_host().levels ? (double)S.sigma_host().n_if[0] : 0.0),
+                    (G && G->levels) ? (double)G->lv[0].n_int : 0.0, (double)nsub,
                     G ? G->bottom_rho : -1.0, G ? (double)G->bottom_direct : 0.0, G ? (double)G->bottom_k : 0.0,
                     G ? G->bottom_seconds : 0.0, G ? (double)G->bottom_steps : 0.0, G ? G->bottom_check : -1.0};
     for (int i = 0; i < n && i < 13; ++i) out[i] = v[i];
@@ -341,13 +422,13 @@ static void plan_stats_impl(const SigmaHost& H, int level, int which, double* ou
 
 hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out, int cap, int* heights) {
     if (!ctx || !heights || cap < 0 || (cap > 0 && !out) || (which != 0 && which != 1)) return HDD_ERR_ARG;
-    HDD_TRY(ctx, plan_stats_impl(ctx->solver->sigma_host, level, which, out, cap, heights));
+    HDD_TRY(ctx, plan_stats_impl(ctx->solver->sigma_host(), level, which, out, cap, heights));
 }
 
 hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max) {
     if (!ctx || !k || cap < 0 || (cap > 0 && !lambdas) || !(lambda_cut > 0.0 && lambda_cut < 1.0)) return HDD_ERR_ARG;
     HDD_TRY(ctx, {
-        const SigmaHost& H = ctx->solver->sigma_host;
+        const SigmaHost& H = ctx->solver->sigma_host();
         if (H.levels == 0) throw std::runtime_error("no sigma levels");
         const int l = H.levels - 1;
         Csr C; std::vector<Int3> xyz;

@@ -34,10 +34,24 @@ void launch_prolong(const DevLevel& F, const DevLevel& C, float* u_f, const floa
 void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s);
 void launch_project(const DevLevel& L, float* v, hipStream_t s);
 void launch_sub_project(const DevLevel& L, const float* b, const float* q, float* r, hipStream_t s);
-void launch_copy2(float* out, const float* in1, const float* in2, const float* cptr, float csign, size_t n, hipStream_t s);
-void launch_xr_update(float* x, float* r, const float* sv, const float* q, const float* alpha, size_t n, hipStream_t s);
 int reduce_parts(size_t n);
-void launch_dot(const float* a, const float* b, size_t n, double* part, int mode, float* scal, double* dscal, hipStream_t s);
-void launch_maxabs(const float* a, size_t n, float* part, float* scal, hipStream_t s);
+
+// Per-problem view of the level-0 cells of a batch (see the PCG kernels in hdd_kernels.hip).
+struct DevSegments {
+    int nprob = 0, max_parts = 1;
+    size_t ncells = 0;                // cells of the real pages (the zero page excluded)
+    const int* pg0 = nullptr;         // [nprob+1] first page of each problem
+    const int* parts = nullptr;       // [nprob] reduction blocks of each problem
+    const int* page_prob = nullptr;   // [pages+1] problem of each page, -1 for the zero page
+    const int* active = nullptr;      // [nprob] 1 while the problem iterates
+    double* dpart = nullptr;          // [nprob*max_parts]
+    float* fpart = nullptr;           // [nprob*max_parts]
+    float* scal = nullptr;            // [nprob*8]
+    double* dscal = nullptr;          // [nprob*4]
+};
+void launch_seg_copy2(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s);
+void launch_seg_xr_update(const DevSegments& S, float* x, float* r, const float* sv, const float* q, hipStream_t s);
+void launch_seg_dot(const DevSegments& S, const float* a, const float* b, int mode, hipStream_t s);
+void launch_seg_maxabs(const DevSegments& S, const float* a, hipStream_t s);
 
 }  // namespace hdd

@@ -169,60 +169,89 @@ __global__ void __launch_bounds__(256) k_sub_project(DevLevel L, const float* __
     r[e] = (L.flags[e] & (kActive | kInterface)) ? b[e] - q[e] : 0.f;
 }
 
-// out = in1 * c + in2 (SPGrid_Copy2), c read from device scalar slot
-__global__ void __launch_bounds__(256) k_copy2(float* __restrict__ out, const float* __restrict__ in1, const float* __restrict__ in2,
-                                               const float* __restrict__ cptr, float csign, size_t n) {
-    const float cc = csign * cptr[0];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in1[i] * cc + in2[i];
-}
+// ---------------------------------------------------------------------------
+// PCG vector operations, per problem. Problem p of a batch owns the level-0
+// pages [pg0[p], pg0[p+1]); page_prob maps a page to its problem (-1 for the
+// zero page, which is never written). Each problem's reductions are laid out
+// exactly as a one-problem reduction over its own (pages + 1) * 256 cells
+// (parts[p] blocks striding over them; the zero page adds nothing), so a
+// problem's sums and scalars do not depend on the batch it is solved in.
+// Scalars: 8 floats per problem (rho, alpha, beta, rho_old, max|r|), 4 doubles.
 
-// x = s*alpha + x ; r = q*(-alpha) + r  (two SPGrid_Copy2 calls fused)
-__global__ void __launch_bounds__(256) k_xr_update(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ s,
-                                                   const float* __restrict__ q, const float* __restrict__ alpha, size_t n) {
-    const float a = alpha[0], na = -a;
+// out = in1 * scal[slot] + in2 (SPGrid_Copy2 with the problem's scalar), active problems only
+__global__ void __launch_bounds__(256) k_seg_copy2(float* __restrict__ out, const float* __restrict__ in1, const float* __restrict__ in2,
+                                                   const float* __restrict__ scal, int slot, const int* __restrict__ page_prob,
+                                                   const int* __restrict__ active, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int pr = page_prob[i >> 8];
+        if (pr < 0 || !active[pr]) continue;
+        out[i] = in1[i] * scal[pr * 8 + slot] + in2[i];
+    }
+}
+
+// x = s*alpha + x ; r = q*(-alpha) + r  (two SPGrid_Copy2 calls fused), active problems only
+__global__ void __launch_bounds__(256) k_seg_xr_update(float* __restrict__ x, float* __restrict__ r, const float* __restrict__ s,
+                                                       const float* __restrict__ q, const float* __restrict__ scal,
+                                                       const int* __restrict__ page_prob, const int* __restrict__ active, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int pr = page_prob[i >> 8];
+        if (pr < 0 || !active[pr]) continue;
+        const float a = scal[pr * 8 + 1], na = -a;
         x[i] = s[i] * a + x[i];
         r[i] = q[i] * na + r[i];
     }
 }
 
-// block-partial double dot products of float products (a*b rounded to float, then summed in double)
-__global__ void __launch_bounds__(256) k_dot_partial(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ part, size_t n) {
+// block-partial double dot products of float products (a*b rounded to float, then summed in double);
+// block (c, p) = partial c of problem p
+__global__ void __launch_bounds__(256) k_seg_dot_partial(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ part,
+                                                         const int* __restrict__ pg0, const int* __restrict__ parts, int max_parts) {
+    const int p = blockIdx.y, c = blockIdx.x, g = parts[p];
+    if (c >= g) return;
+    const size_t base = (size_t)pg0[p] * 256, n = (size_t)(pg0[p + 1] - pg0[p]) * 256;
     __shared__ double sh[4];
     double s = 0.0;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (size_t)gridDim.x * blockDim.x) s += (double)(a[i] * b[i]);
+    for (size_t i = (size_t)c * 256 + threadIdx.x; i < n; i += (size_t)g * 256) s += (double)(a[base + i] * b[base + i]);
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    if (threadIdx.x == 0) part[(size_t)p * max_parts + c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
-__global__ void __launch_bounds__(256) k_maxabs_partial(const float* __restrict__ a, float* __restrict__ part, size_t n) {
+__global__ void __launch_bounds__(256) k_seg_maxabs_partial(const float* __restrict__ a, float* __restrict__ part,
+                                                            const int* __restrict__ pg0, const int* __restrict__ parts, int max_parts) {
+    const int p = blockIdx.y, c = blockIdx.x, g = parts[p];
+    if (c >= g) return;
+    const size_t base = (size_t)pg0[p] * 256, n = (size_t)(pg0[p + 1] - pg0[p]) * 256;
     __shared__ float sh[4];
     float m = 0.f;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (size_t)gridDim.x * blockDim.x) m = fmaxf(m, fabsf(a[i]));
+    for (size_t i = (size_t)c * 256 + threadIdx.x; i < n; i += (size_t)g * 256) m = fmaxf(m, fabsf(a[base + i]));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    if (threadIdx.x == 0) part[(size_t)p * max_parts + c] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
 }
 
-// Final reduction of the partials and the PCG scalar logic of
-// CONJUGATE_GRADIENT_DUMPING::Solve (:193-205) in float:
+// Final reduction of one problem's partials (workgroup p) and the PCG scalar
+// logic of CONJUGATE_GRADIENT_DUMPING::Solve (:193-205) in float:
 //   mode 0: rho = (float)dot -> scal[0]; beta = rho/rho_old -> scal[2]
 //   mode 1: s_dot_q = (float)dot; alpha = rho/s_dot_q -> scal[1]; rho_old = rho -> scal[3]
-//   mode 2: max-abs norm -> scal[4]
-__global__ void __launch_bounds__(256) k_finish(const double* __restrict__ dpart, const float* __restrict__ fpart, int nparts, int mode,
-                                                float* __restrict__ scal, double* __restrict__ dscal) {
+//   mode 2: max-abs norm -> scal[4] (every problem; modes 0 and 1 active problems only)
+__global__ void __launch_bounds__(256) k_seg_finish(const double* __restrict__ dpart, const float* __restrict__ fpart,
+                                                    const int* __restrict__ parts, int max_parts, int mode, const int* __restrict__ active,
+                                                    float* __restrict__ scal_all, double* __restrict__ dscal_all) {
+    const int p = blockIdx.x;
+    if (mode < 2 && !active[p]) return;
+    const int nparts = parts[p];
+    float* scal = scal_all + p * 8;
+    double* dscal = dscal_all + p * 4;
     __shared__ double sd[256];
     __shared__ float sf[256];
     double s = 0.0;
     float m = 0.f;
     for (int i = threadIdx.x; i < nparts; i += 256) {
-        if (mode < 2) s += dpart[i]; else m = fmaxf(m, fpart[i]);
+        if (mode < 2) s += dpart[(size_t)p * max_parts + i]; else m = fmaxf(m, fpart[(size_t)p * max_parts + i]);
     }
     sd[threadIdx.x] = s; sf[threadIdx.x] = m;
     __syncthreads();
@@ -247,7 +276,6 @@ __global__ void __launch_bounds__(256) k_finish(const double* __restrict__ dpart
         }
     }
 }
-
 // ---------------------------------------------------------------------------
 // host launchers
 static inline int grid_for(size_t n) { size_t g = (n + 255) / 256; return (int)(g > 4096 ? 4096 : (g == 0 ? 1 : g)); }
@@ -295,22 +323,24 @@ void launch_project(const DevLevel& L, float* v, hipStream_t s) {
 void launch_sub_project(const DevLevel& L, const float* b, const float* q, float* r, hipStream_t s) {
     if (L.np) hipLaunchKernelGGL(k_sub_project, dim3(L.np), dim3(256), 0, s, L, b, q, r);
 }
-void launch_copy2(float* out, const float* in1, const float* in2, const float* cptr, float csign, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy2, dim3(grid_for(n)), dim3(256), 0, s, out, in1, in2, cptr, csign, n);
-}
-void launch_xr_update(float* x, float* r, const float* sv, const float* q, const float* alpha, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_xr_update, dim3(grid_for(n)), dim3(256), 0, s, x, r, sv, q, alpha, n);
-}
 int reduce_parts(size_t n) { return grid_for(n) > 1024 ? 1024 : grid_for(n); }
-void launch_dot(const float* a, const float* b, size_t n, double* part, int mode, float* scal, double* dscal, hipStream_t s) {
-    const int g = reduce_parts(n);
-    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(256), 0, s, a, b, part, n);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, (const float*)nullptr, g, mode, scal, dscal);
+void launch_seg_copy2(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_copy2, dim3(grid_for(S.ncells)), dim3(256), 0, s, out, in1, in2, (const float*)S.scal, slot, S.page_prob,
+                       S.active, S.ncells);
 }
-void launch_maxabs(const float* a, size_t n, float* part, float* scal, hipStream_t s) {
-    const int g = reduce_parts(n);
-    hipLaunchKernelGGL(k_maxabs_partial, dim3(g), dim3(256), 0, s, a, part, n);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, (const double*)nullptr, part, g, 2, scal, (double*)nullptr);
+void launch_seg_xr_update(const DevSegments& S, float* x, float* r, const float* sv, const float* q, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_xr_update, dim3(grid_for(S.ncells)), dim3(256), 0, s, x, r, sv, q, (const float*)S.scal, S.page_prob,
+                       S.active, S.ncells);
+}
+void launch_seg_dot(const DevSegments& S, const float* a, const float* b, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_dot_partial, dim3(S.max_parts, S.nprob), dim3(256), 0, s, a, b, S.dpart, S.pg0, S.parts, S.max_parts);
+    hipLaunchKernelGGL(k_seg_finish, dim3(S.nprob), dim3(256), 0, s, (const double*)S.dpart, (const float*)nullptr, S.parts, S.max_parts,
+                       mode, S.active, S.scal, S.dscal);
+}
+void launch_seg_maxabs(const DevSegments& S, const float* a, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_maxabs_partial, dim3(S.max_parts, S.nprob), dim3(256), 0, s, a, S.fpart, S.pg0, S.parts, S.max_parts);
+    hipLaunchKernelGGL(k_seg_finish, dim3(S.nprob), dim3(256), 0, s, (const double*)nullptr, (const float*)S.fpart, S.parts, S.max_parts,
+                       2, S.active, S.scal, S.dscal);
 }
 
 }  // namespace hdd

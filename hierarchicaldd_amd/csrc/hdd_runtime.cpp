@@ -1,5 +1,6 @@
 #include "hdd_runtime.h"
 #include "hdd_sigma_device.h"
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -13,76 +14,192 @@ DevLevel LevelBuffers::view() const {
     return v;
 }
 
-void Solver::build_levels(int n_pages, const uint64_t* page_offsets, const uint32_t* flags) {
-    lv.clear();
+ProblemHost::ProblemHost(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* bin)
+    : P(p) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (P.vlevels < 1) throw std::runtime_error("vlevels must be >= 1");
     lv.resize(P.vlevels);
-    PagedLevel& L0 = lv[0].host;
+    PagedLevel& L0 = lv[0];
     L0.init(P.size);
     for (int i = 0; i < n_pages; ++i) {
         int bx, by, bz;
         page_coords(page_offsets[i] >> 12, bx, by, bz);
-        const int p = L0.add_page(bx, by, bz);
-        std::copy(flags + (size_t)i * kPageCells, flags + (size_t)(i + 1) * kPageCells, L0.flags.begin() + (size_t)p * kPageCells);
+        const int q = L0.add_page(bx, by, bz);
+        std::copy(flags + (size_t)i * kPageCells, flags + (size_t)(i + 1) * kPageCells, L0.flags.begin() + (size_t)q * kPageCells);
     }
     L0.sort_pages();
+    // rhs b in page order of level 0 (input order may differ: remap through the LUT)
+    input_page.resize(n_pages);
+    b.assign((size_t)L0.npages() * kPageCells, 0.f);
+    for (int i = 0; i < n_pages; ++i) {
+        int bx, by, bz;
+        page_coords(page_offsets[i] >> 12, bx, by, bz);
+        const int q = L0.page_at_block(bx, by, bz);
+        input_page[i] = q;
+        std::copy(bin + (size_t)i * kPageCells, bin + (size_t)(i + 1) * kPageCells, b.begin() + (size_t)q * kPageCells);
+    }
+    // the interface operators are generated from the input flags, before division
+    // (Initialize_Preconditioner: Split precedes Domain_Divider, DDS/...Solver.h:122-133)
+    generate_interface_operators(L0, P.size, P.sub, P.sigma_levels, sigma);
+    divide_subdomains(L0, P.size, P.sub);
+    set_face_flags(L0);
+    for (int l = 0; l + 1 < P.vlevels; ++l) {
+        const Int3 s = lv[l].size;
+        lv[l + 1].init({s.x / 2 + 1, s.y / 2 + 1, s.z / 2 + 1});
+        downsample_level(lv[l], lv[l + 1]);
+    }
+    for (int l = 0; l < P.vlevels; ++l) {
+        set_face_flags(lv[l]);
+        set_boundary_flags(lv[l]);
+    }
+    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+ProblemHost::~ProblemHost() {}
+
+void ProblemHost::factorize() {
+    if (factors) return;
+    auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<SigmaFactors> f(new SigmaFactors());
+    factor_sigma(sigma, *f);
+    factors = std::move(f);
+    factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+size_t ProblemHost::device_bytes() const {
+    // every level: flags, two smoothing buffers, b, r (4 B each per slot), neighbour and origin tables
+    double bytes = 0;
+    for (const PagedLevel& L : lv) bytes += (L.npages() + 1.0) * kPageCells * 20.0 + L.npages() * 36.0;
+    bytes += (lv[0].npages() + 1.0) * kPageCells * 16.0;  // x, b, q, s
+    // interface solver: float plans of the factors (diagonal inverse twice, L21 once), operators, vectors
+    for (int l = 0; l < sigma.levels; ++l) {
+        for (const Csr& A : sigma.Air[l]) bytes += 16.0 * A.nnz();
+        bytes += 8.0 * sigma.Arr[l].nnz() + 64.0 * sigma.n_if[l];
+        if (l + 1 < sigma.levels) bytes += 16.0 * sigma.Prr[l].nnz();
+        if (factors && l < (int)factors->lv.size()) {
+            const SigmaFactors::Level& F = factors->lv[l];
+            for (const SupernodalFactor& f : F.Aii) bytes += 8.0 * f.nnz();
+            bytes += 8.0 * F.Arr.nnz();
+        } else {
+            for (const Csr& A : sigma.Aii[l]) bytes += 64.0 * A.nnz();
+            bytes += 64.0 * sigma.Arr[l].nnz();
+        }
+    }
+    if (factors) bytes += 8.0 * factors->bottom.nnz() + 8.0 * factors->spec.V.size();
+    return (size_t)(bytes * 1.1) + (size_t(4) << 20);
 }
 
 Solver::Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b, bool host)
     : P(p), host_only(host) {
     auto t0 = std::chrono::steady_clock::now();
-    if (P.vlevels < 1) throw std::runtime_error("vlevels must be >= 1");
-    build_levels(n_pages, page_offsets, flags);
-    // the interface operators are generated from the input flags, before division
-    // (Initialize_Preconditioner: Split precedes Domain_Divider, DDS/...Solver.h:122-133)
-    generate_interface_operators(lv[0].host, P.size, P.sub, P.sigma_levels, sigma_host);
-    divide_subdomains(lv[0].host, P.size, P.sub);
-    set_face_flags(lv[0].host);
-    for (int l = 0; l + 1 < P.vlevels; ++l) {
-        const Int3 s = lv[l].host.size;
-        lv[l + 1].host.init({s.x / 2 + 1, s.y / 2 + 1, s.z / 2 + 1});
-        downsample_level(lv[l].host, lv[l + 1].host);
+    prob.push_back(std::make_shared<ProblemHost>(p, n_pages, page_offsets, flags, b));
+    if (host_only) merge_levels();
+    else build_device();
+    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+Solver::Solver(const std::vector<std::shared_ptr<ProblemHost>>& problems) : prob(problems) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (prob.empty()) throw std::runtime_error("a batch needs at least one problem");
+    P = prob[0]->P;
+    for (auto& q : prob) {
+        const SolverParams& o = q->P;
+        if (o.vlevels != P.vlevels || o.sigma_levels != P.sigma_levels || o.nV != P.nV || o.int_s != P.int_s || o.bnd_s != P.bnd_s ||
+            o.sig_s != P.sig_s || o.sig_mg != P.sig_mg)
+            throw std::runtime_error("every problem of a batch needs the same V-cycle and sigma schedule (levels and smoothing counts)");
     }
-    for (int l = 0; l < P.vlevels; ++l) {
-        set_face_flags(lv[l].host);
-        set_boundary_flags(lv[l].host);
+    build_device();
+    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Problems side by side along x: offsets are multiples of one coarsest-level
+// page (4 * 2^(vlevels-1) cells) with at least that much empty space between
+// consecutive boxes, so at every level no page holds cells of two problems.
+void Solver::merge_levels() {
+    const int NP = nprob(), NL = P.vlevels;
+    const int align = kPX << (NL - 1);
+    xoff.assign(NP, 0);
+    for (int p = 1; p < NP; ++p) {
+        const int end = xoff[p - 1] + prob[p - 1]->lv[0].size.x + align;
+        xoff[p] = (end + align - 1) / align * align;
     }
-    if (host_only) {
-        setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        return;
+    lv.clear();
+    lv.resize(NL);
+    pg0.assign(NL, std::vector<int>(NP + 1, 0));
+    for (int l = 0; l < NL; ++l) {
+        Int3 ext{0, 0, 0};
+        for (int p = 0; p < NP; ++p) {
+            const PagedLevel& S = prob[p]->lv[l];
+            ext.x = std::max(ext.x, (xoff[p] >> l) + S.size.x);
+            ext.y = std::max(ext.y, S.size.y);
+            ext.z = std::max(ext.z, S.size.z);
+        }
+        PagedLevel& M = lv[l].host;
+        M.init(ext);
+        for (int p = 0; p < NP; ++p) {
+            pg0[l][p] = M.npages();
+            const PagedLevel& S = prob[p]->lv[l];
+            const int dx = xoff[p] >> l;
+            for (int q = 0; q < S.npages(); ++q) {
+                const Int3 o = S.page_base[q];
+                const int m = M.add_page((o.x + dx) >> 2, o.y >> 3, o.z >> 3);
+                if (m != M.npages() - 1) throw std::runtime_error("batch layout: pages of two problems overlap");
+                std::copy(S.flags.begin() + (size_t)q * kPageCells, S.flags.begin() + (size_t)(q + 1) * kPageCells,
+                          M.flags.begin() + (size_t)m * kPageCells);
+            }
+        }
+        pg0[l][NP] = M.npages();
     }
+}
+
+void Solver::build_device() {
+    const int NP = nprob();
+    for (auto& q : prob) q->factorize();
     hdd_check(hipStreamCreate(&stream), "hipStreamCreate");
     upload_constants();
+    merge_levels();
     upload_levels();
-    // rhs b in page order of level 0 (input order may differ: remap through the LUT)
+    const PagedLevel& L0 = lv[0].host;
+    const int np0 = L0.npages();
     {
-        const PagedLevel& L0 = lv[0].host;
-        std::vector<float> hb((size_t)(L0.npages() + 1) * kPageCells, 0.f);
-        for (int i = 0; i < n_pages; ++i) {
-            int bx, by, bz;
-            page_coords(page_offsets[i] >> 12, bx, by, bz);
-            const int q = L0.page_at_block(bx, by, bz);
-            std::copy(b + (size_t)i * kPageCells, b + (size_t)(i + 1) * kPageCells, hb.begin() + (size_t)q * kPageCells);
-        }
+        std::vector<float> hb((size_t)(np0 + 1) * kPageCells, 0.f);
+        for (int p = 0; p < NP; ++p) std::copy(prob[p]->b.begin(), prob[p]->b.end(), hb.begin() + (size_t)pg0[0][p] * kPageCells);
         bvec.upload(hb);
     }
     const size_t n0 = lv[0].cells();
     x.alloc(n0); q.alloc(n0); s.alloc(n0);
-    dpart.alloc(1024); dscal.alloc(4); fpart.alloc(1024); scal.alloc(8);
-    // interface dof -> level-0 cell index
+    // per-problem reductions: each problem's partials as for a one-problem grid of its own size
     {
-        const PagedLevel& L0 = lv[0].host;
-        std::vector<int> idx(sigma_host.interface_map.size());
-        for (size_t i = 0; i < idx.size(); ++i) {
-            const Int3 c = sigma_host.interface_map[i];
-            const int pg = L0.page_at(c.x, c.y, c.z);
-            const bool ok = pg >= 0 && (L0.flags[(size_t)pg * kPageCells + page_cell(c.x, c.y, c.z)] & kInterface);
-            idx[i] = ok ? pg * kPageCells + page_cell(c.x, c.y, c.z) : -1;
+        std::vector<int> parts(NP), pp((size_t)np0 + 1, -1);
+        max_parts = 1;
+        for (int p = 0; p < NP; ++p) {
+            parts[p] = reduce_parts((size_t)(pg0[0][p + 1] - pg0[0][p] + 1) * kPageCells);
+            max_parts = std::max(max_parts, parts[p]);
+            for (int g = pg0[0][p]; g < pg0[0][p + 1]; ++g) pp[g] = p;
+        }
+        seg_pg0.upload(pg0[0]); seg_parts.upload(parts); page_prob.upload(pp);
+        active.alloc(NP);
+        dpart.alloc((size_t)NP * max_parts); fpart.alloc((size_t)NP * max_parts);
+        scal.alloc((size_t)NP * 8); dscal.alloc((size_t)NP * 4);
+    }
+    // interface dof -> level-0 cell index (problems' dofs concatenated in problem order)
+    {
+        std::vector<int> idx;
+        for (int p = 0; p < NP; ++p) {
+            for (const Int3& c : prob[p]->sigma.interface_map) {
+                const int X = c.x + xoff[p];
+                const int pg = L0.page_at(X, c.y, c.z);
+                const bool ok = pg >= 0 && (L0.flags[(size_t)pg * kPageCells + page_cell(X, c.y, c.z)] & kInterface);
+                idx.push_back(ok ? pg * kPageCells + page_cell(X, c.y, c.z) : -1);
+            }
         }
         if_index.upload(idx);
     }
-    sigma = make_sigma_device(sigma_host, P.sig_s, P.sig_mg);
+    std::vector<const SigmaHost*> hosts;
+    std::vector<const SigmaFactors*> facs;
+    for (auto& q : prob) { hosts.push_back(&q->sigma); facs.push_back(q->factors.get()); }
+    sigma = make_sigma_device(hosts, facs, P.sig_s, P.sig_mg);
     hdd_check(hipDeviceSynchronize(), "setup sync");
-    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 Solver::~Solver() {
@@ -188,46 +305,84 @@ void Solver::interface_scatter() {
 }
 
 // CONJUGATE_GRADIENT_DUMPING::Solve (DDS/CONJUGATE_GRADIENT_DUMPING.cpp:146-212)
-// with the stopping rule max|r| <= tol_abs, x starting at zero.
-int Solver::pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds) {
+// for every problem of the batch, x starting at zero, with the stopping rule
+// max|r| <= tol[p]. Each problem has its own scalars; once it stops, its x
+// and r are frozen (the shared launches still visit its pages).
+void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out) {
+    const int NP = nprob();
+    if ((int)tol.size() != NP || (int)max_it.size() != NP) throw std::runtime_error("pcg: one tolerance and iteration limit per problem");
     LevelBuffers& L0 = lv[0];
     const DevLevel v0 = L0.view();
     const size_t n0 = L0.cells();
     float* r = L0.b.p;
-    hist.clear();
+    DevSegments sg;
+    sg.nprob = NP; sg.max_parts = max_parts; sg.ncells = (size_t)L0.host.npages() * kPageCells;
+    sg.pg0 = seg_pg0.p; sg.parts = seg_parts.p; sg.page_prob = page_prob.p; sg.active = active.p;
+    sg.dpart = dpart.p; sg.fpart = fpart.p; sg.scal = scal.p; sg.dscal = dscal.p;
+    out.assign(NP, PcgResult());
     const float flt_max = 3.40282347e+38f;
-    float init[8] = {0, 0, 0, flt_max, 0, 0, 0, 0};
-    hdd_check(hipMemcpyAsync(scal.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
+    std::vector<float> hs((size_t)NP * 8, 0.f);
+    for (int p = 0; p < NP; ++p) hs[(size_t)p * 8 + 3] = flt_max;  // rho_old
+    std::vector<int> act(NP, 1);
+    hdd_check(hipMemcpyAsync(scal.p, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
     launch_fill(x.p, 0.f, n0, stream);
     hdd_check(hipStreamSynchronize(stream));
     auto t0 = std::chrono::steady_clock::now();
-    const float small_number = 1.1920929e-07f;
-    int it = 0;
-    for (it = 0;; ++it) {
+    int n_active = NP;
+    for (int it = 0;; ++it) {
         const bool restart = (it == 0);
         if (restart) {
             launch_laplace_pcg(v0, x.p, q.p, stream);
             launch_sub_project(v0, bvec.p, q.p, r, stream);
         }
-        launch_maxabs(r, n0, fpart.p, scal.p, stream);
-        float conv = 0.f;
-        hdd_check(hipMemcpyAsync(&conv, scal.p + 4, sizeof(float), hipMemcpyDeviceToHost, stream));
+        launch_seg_maxabs(sg, r, stream);
+        hdd_check(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
         hdd_check(hipStreamSynchronize(stream));
-        hist.push_back((double)conv);
-        if (conv <= (float)tol_abs && (it >= 0 || conv < small_number)) break;
-        if (it == max_it) { it = -it; break; }
+        const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        bool changed = false;
+        for (int p = 0; p < NP; ++p) {
+            if (!act[p]) continue;
+            const float conv = hs[(size_t)p * 8 + 4];
+            out[p].hist.push_back((double)conv);
+            if (conv <= (float)tol[p]) out[p].iterations = it;
+            else if (it >= max_it[p]) out[p].iterations = -it;
+            else continue;
+            out[p].seconds = now;
+            act[p] = 0;
+            --n_active;
+            changed = true;
+        }
+        if (n_active == 0) break;
+        // act is not modified again before the next synchronisation
+        if (changed) hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
         apply_preconditioner();
         float* z = L0.u[L0.cur].p;
-        launch_dot(z, r, n0, dpart.p, 0, scal.p, dscal.p, stream);          // rho, beta
+        launch_seg_dot(sg, z, r, 0, stream);                   // rho, beta
         if (restart) launch_copy(s.p, z, n0, stream);
-        else launch_copy2(s.p, s.p, z, scal.p + 2, 1.f, n0, stream);         // s = s*beta + z
-        launch_laplace_pcg(v0, s.p, q.p, stream);                          // q = Project(A s)
-        launch_dot(s.p, q.p, n0, dpart.p, 1, scal.p, dscal.p, stream);      // alpha, rho_old
-        launch_xr_update(x.p, r, s.p, q.p, scal.p + 1, n0, stream);
+        else launch_seg_copy2(sg, s.p, s.p, z, 2, stream);     // s = s*beta + z
+        launch_laplace_pcg(v0, s.p, q.p, stream);              // q = Project(A s)
+        launch_seg_dot(sg, s.p, q.p, 1, stream);               // alpha, rho_old
+        launch_seg_xr_update(sg, x.p, r, s.p, q.p, stream);
     }
     hdd_check(hipStreamSynchronize(stream));
-    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    return it;
+}
+
+int Solver::pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds) {
+    std::vector<PcgResult> res;
+    pcg_batch(std::vector<double>(nprob(), tol_abs), std::vector<int>(nprob(), max_it), res);
+    hist = res[0].hist;
+    double worst = 0;
+    for (auto& r : res) worst = std::max(worst, r.seconds);
+    if (seconds) *seconds = worst;
+    return res[0].iterations;
+}
+
+void Solver::download_x(int p, float* host) {
+    if (p < 0 || p >= nprob()) throw std::runtime_error("no such problem in the batch");
+    hdd_check(hipStreamSynchronize(stream), "sync");
+    const size_t first = (size_t)pg0[0][p] * kPageCells, n = (size_t)(pg0[0][p + 1] - pg0[0][p]) * kPageCells;
+    hdd_check(hipMemcpy(host, x.p + first, n * sizeof(float), hipMemcpyDeviceToHost), "download x");
 }
 
 }  // namespace hdd

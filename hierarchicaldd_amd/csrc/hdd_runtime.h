@@ -1,6 +1,21 @@
 // Product runtime: the host facade that mirrors the reference's
 // SPGrid_Domain_Decomposition_Solver (DDS/SPGrid_Domain_Decomposition_Solver.h:24-198)
 // and runs every hot-path stage on the GPU.
+//
+// A Solver holds one or more independent problems in one device layout, so
+// that every stage of the preconditioner is one launch for all of them:
+//  * V-cycle levels: the problems' pages side by side in one paged grid
+//    (problem p translated by xoff[p] cells along x at level 0, xoff[p] >> l
+//    at level l; offsets are multiples of 4 * 2^(vlevels-1), so every level's
+//    page boundaries and the coarse map c = (f+1)/2 are preserved, and a gap
+//    keeps the problems' pages apart). No stencil, restriction or
+//    prolongation reads across problems: face bits are per problem, and the
+//    transfer stencils of Active cells stay inside the problem's own box.
+//  * Interface solver: the problems' operators stacked block-diagonally.
+//  * PCG: per-problem scalars (rho, alpha, beta), dot products and L-infinity
+//    norms reduced per problem, and a per-problem stopping test; a converged
+//    problem's x and r are frozen while the others continue.
+// One problem is the special case of a batch of one.
 #pragma once
 #include <memory>
 #include <vector>
@@ -38,6 +53,28 @@ struct DBuf {
     }
 };
 
+struct SigmaFactors;  // hdd_sigma_device.h
+
+// Host half of Read_SPGrid + Initialize_Preconditioner for one problem
+// (DDS/SPGrid_Domain_Decomposition_Solver.h:73-161): page pool, subdomain
+// division, flags of every V-cycle level, interface operators, and (after
+// factorize()) the factors of the interface solver's exact solves.
+struct ProblemHost {
+    SolverParams P;
+    std::vector<PagedLevel> lv;        // V-cycle levels; lv[0] is the input grid (Morton page order)
+    SigmaHost sigma;
+    std::vector<int> input_page;       // level-0 page of each input page
+    std::vector<float> b;              // level-0 right-hand side, npages*256 in page order
+    std::unique_ptr<SigmaFactors> factors;
+    double setup_seconds = 0, factor_seconds = 0;
+
+    ProblemHost(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b);
+    ~ProblemHost();
+    void factorize();                  // idempotent
+    // device bytes of the problem inside a batch (pages of every level, vectors, interface plans)
+    size_t device_bytes() const;
+};
+
 struct LevelBuffers {
     PagedLevel host;
     DBuf<uint32_t> flags;
@@ -48,34 +85,53 @@ struct LevelBuffers {
     size_t cells() const { return (size_t)(host.npages() + 1) * kPageCells; }
 };
 
+struct PcgResult {
+    int iterations = 0;                // negative: not converged within max_it
+    double seconds = 0;                // from the start of the batch solve to this problem's last test
+    std::vector<double> hist;          // max|r| per iteration
+};
+
 class Solver {
 public:
-    SolverParams P;
-    std::vector<LevelBuffers> lv;   // V-cycle levels; lv[0] is the main grid
-    DBuf<float> x, bvec, q, s;      // second-allocator channels (x, b, q, s)
+    SolverParams P;                    // schedule shared by every problem
+    std::vector<std::shared_ptr<ProblemHost>> prob;
+    std::vector<int> xoff;             // level-0 x offset of each problem in the shared grid
+    std::vector<std::vector<int>> pg0; // [level][problem] first page; pg0[l][nprob] = pages of level l
+    std::vector<LevelBuffers> lv;      // V-cycle levels of the shared grid; lv[0] is the main grid
+    DBuf<float> x, bvec, q, s;         // second-allocator channels (x, b, q, s)
     DBuf<double> dpart, dscal;
-    DBuf<float> fpart, scal;        // scal: rho, alpha, beta, rho_old, max|r|
-    SigmaHost sigma_host;
+    DBuf<float> fpart, scal;           // scal: 8 per problem (rho, alpha, beta, rho_old, max|r|)
+    DBuf<int> seg_pg0, seg_parts, page_prob, active;
+    int max_parts = 1;
     std::unique_ptr<SigmaDevice> sigma;
-    DBuf<int> if_index;             // level-0 cell index of each interface dof
+    DBuf<int> if_index;                // level-0 cell index of each interface dof
     hipStream_t stream = nullptr;
     double setup_seconds = 0;
+    bool host_only = false;
 
-    // host_only: run the host setup and skip every device allocation (introspection)
+    // one problem; host_only: run the host setup and skip every device allocation (introspection)
     Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b,
            bool host_only = false);
-    bool host_only = false;
+    // a batch of problems set up on the host (factorised here if they are not yet)
+    explicit Solver(const std::vector<std::shared_ptr<ProblemHost>>& problems);
     ~Solver();
 
+    int nprob() const { return (int)prob.size(); }
+    const SigmaHost& sigma_host() const { return prob[0]->sigma; }
     // level-0 channel access in page order (c: 0=z 1=r 2=k 3=x 4=b 5=q 6=s)
     float* channel(int c);
     void vcycle();
     void smooth0(int mode);
     void apply_preconditioner();  // z <- M^{-1} r
     int pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds);
+    // every problem from x = 0; tol[p], max_it[p] per problem
+    void pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out);
+    // x of problem p (its level-0 pages in its own page order), n_pages(p)*256 floats
+    void download_x(int p, float* host);
 
 private:
-    void build_levels(int n_pages, const uint64_t* page_offsets, const uint32_t* flags);
+    void build_device();
+    void merge_levels();
     void upload_levels();
     void sweeps(int l, int count, bool interior, bool boundary);
     void interface_gather();

@@ -6,6 +6,7 @@
 #include "hdd_factor.h"
 #include "hdd_runtime.h"
 #include "hdd_sigma.h"
+#include "hdd_spectral.h"
 #include "hdd_triplan.h"
 
 namespace hdd {
@@ -41,24 +42,55 @@ struct SigmaLevel {
     DBuf<float> u, b, r, vb, vx, tmp;
 };
 
+// Host factors of one problem's interface solver: the exact solves of
+// SIGMA_SMOOTHER (IFS/SIGMA_SMOOTHER.cpp:115-288, PARDISO phases 11/22 in the
+// reference) and the bottom level's combined system and slow eigenmodes.
+struct SigmaFactors {
+    struct Level {
+        std::vector<int> rep;                  // subdomain -> first subdomain with an identical Aii
+        std::vector<SupernodalFactor> Aii;     // factors of the representatives (others empty)
+        SupernodalFactor Arr;
+    };
+    std::vector<Level> lv;
+    // Bottom level: the 200 smooths of the reference as one exact solve of the
+    // combined system [[Aii+eI, Air],[Air^T, Arr+eI]] plus a low-rank
+    // correction on the eigenmodes of the smoother that survive 200 sweeps.
+    bool bottom_direct = false;                // the combined solve replaces the sweeps
+    int bottom_n_int = 0;                      // interior rows of the combined system
+    SupernodalFactor bottom;                   // its factor
+    BottomSpectrum spec;                       // kept modes (none when the correction is off)
+    double bottom_rho = -1;                    // largest eigenvalue of the bottom smoother (-1: not computed)
+    int bottom_steps = 0;
+    double bottom_seconds = 0;
+    double seconds = 0;
+};
+void factor_sigma(const SigmaHost& H, SigmaFactors& F);
+
 struct SigmaDevice {
-    int levels = 0, smoothing = 3, n_mg = 1;
+    int levels = 0, smoothing = 3, n_mg = 1, nprob = 1;
     // Bottom level: the reference runs 200 smooths, a stationary iteration
     // u <- (Arr+eI)^-1 (b + K u) whose iteration matrix M has eigenvalues in
     // [0, 1). Its result is x* + M^200 (u0 - x*) with x* the solution of the
     // bottom Schur system, computed here by one solve of the combined system
     // [[Aii+eI, Air], [Air^T, Arr+eI]], and M^200 kept on the k eigenmodes whose
     // 200th power is above 1e-9 (V, S V and their weights, found at setup).
+    // In a batch every problem's combined system is one block of the plan and
+    // every problem keeps its own modes; the shortcut is used only when every
+    // problem of the batch resolved its spectrum.
     bool bottom_direct = false;
-    double bottom_rho = -1;        // largest eigenvalue of M (-1: not computed)
-    int bottom_k = 0;              // eigenmodes kept
-    int bottom_steps = 0;          // block Krylov steps of the eigen-solve
-    double bottom_seconds = 0;     // host time of the eigen-solve
+    double bottom_rho = -1;        // largest eigenvalue of M over the batch (-1: not computed)
+    int bottom_k = 0;              // eigenmodes kept (all problems)
+    int bottom_steps = 0;          // block Krylov steps of the eigen-solves (largest)
+    double bottom_seconds = 0;     // host time of the eigen-solves
     double bottom_check = -1;      // max |direct+correction - 200 sweeps| / max |200 sweeps| on a random rhs
     TriPlan Lbottom;
     DBuf<float> bot_rhs, bot_sol, spec_V, spec_SV, spec_coef;
     DBuf<double> spec_w1, spec_w2, spec_part;
-    int spec_nch = 0;
+    DBuf<int> bot_pos;             // bottom interface dof -> row of the combined systems
+    DBuf<int> row_prob;            // bottom interface dof -> problem
+    DBuf<int> mode_row0, mode_len, prob_m0, prob_row0, prob_n;
+    DBuf<long long> mode_vofs, prob_vofs;
+    int spec_nch = 0;              // chunks of the longest problem
     void bottom(hipStream_t s, bool u_zero);
     std::vector<SigmaLevel> lv;
     float* b0 = nullptr;   // level-0 interface rhs (lv[0].b)
@@ -71,7 +103,10 @@ struct SigmaDevice {
     void vcycle(hipStream_t s, bool first);
 };
 
-std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& host, int smoothing, int n_mg);
+// Device interface solver of a batch: the problems' operators stacked
+// block-diagonally (problem order), from their host operators and factors.
+std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost*>& hosts, const std::vector<const SigmaFactors*>& factors,
+                                               int smoothing, int n_mg);
 void sigma_vcycles(SigmaDevice& S, hipStream_t s);
 void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s);
 void launch_interface_scatter(SigmaDevice& S, const int* if_index, float* z, hipStream_t s);

@@ -331,19 +331,29 @@ __global__ void k_prolong_add(int rows, const int* __restrict__ p, const int* __
     u[i] += acc;
 }
 
-// Low-rank part of the bottom solve: part[j][c] = (V_j . b, (S V)_j . u) over
-// rows [c*kSpecChunk, (c+1)*kSpecChunk), in double; u == nullptr means u = 0.
+// Low-rank part of the bottom solve. Modes of all problems are numbered
+// together; mode j belongs to the problem whose bottom interface dofs are
+// [mode_row0[j], mode_row0[j] + mode_len[j]), and its vectors V_j and (S V)_j
+// hold mode_len[j] entries at mode_vofs[j].
+// part[j][c] = (V_j . b, (S V)_j . u) over rows [c*kSpecChunk, (c+1)*kSpecChunk)
+// of the mode's problem, in double; u == nullptr means u = 0.
 constexpr int kSpecChunk = 4096;
-__global__ void __launch_bounds__(256) k_spec_dot(int n, int nch, const float* __restrict__ V, const float* __restrict__ SV,
-                                                  const float* __restrict__ b, const float* __restrict__ u, double* __restrict__ part) {
+__global__ void __launch_bounds__(256) k_spec_dot(int nch, const int* __restrict__ mode_row0, const int* __restrict__ mode_len,
+                                                  const long long* __restrict__ mode_vofs, const float* __restrict__ V,
+                                                  const float* __restrict__ SV, const float* __restrict__ b, const float* __restrict__ u,
+                                                  double* __restrict__ part) {
     const int c = blockIdx.x, j = blockIdx.y;
+    const int n = mode_len[j];
     const int r0 = c * kSpecChunk, r1 = min(n, r0 + kSpecChunk);
-    const float* vj = V + (size_t)j * n;
+    if (r0 >= n) return;
+    const float* vj = V + mode_vofs[j];
+    const float* bj = b + mode_row0[j];
     double sb = 0.0, su = 0.0;
-    for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) sb += (double)vj[i] * (double)b[i];
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) sb += (double)vj[i] * (double)bj[i];
     if (u) {
-        const float* sj = SV + (size_t)j * n;
-        for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) su += (double)sj[i] * (double)u[i];
+        const float* sj = SV + mode_vofs[j];
+        const float* uj = u + mode_row0[j];
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) su += (double)sj[i] * (double)uj[i];
     }
     for (int o = 32; o > 0; o >>= 1) { sb += __shfl_down(sb, o); su += __shfl_down(su, o); }
     __shared__ double red[2][4];
@@ -357,23 +367,39 @@ __global__ void __launch_bounds__(256) k_spec_dot(int n, int nch, const float* _
 }
 
 // coef_j = l_j^200 (v_j^T S u - v_j^T b / (1 - l_j)) = w2_j * su_j - w1_j * sb_j
-__global__ void k_spec_coef(int k, int nch, const double* __restrict__ part, const double* __restrict__ w1, const double* __restrict__ w2,
-                            float* __restrict__ coef) {
+__global__ void k_spec_coef(int k, int nch, const int* __restrict__ mode_len, const double* __restrict__ part, const double* __restrict__ w1,
+                            const double* __restrict__ w2, float* __restrict__ coef) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= k) return;
+    const int ncj = (mode_len[j] + kSpecChunk - 1) / kSpecChunk;
     double sb = 0.0, su = 0.0;
-    for (int c = 0; c < nch; ++c) { sb += part[((size_t)j * nch + c) * 2]; su += part[((size_t)j * nch + c) * 2 + 1]; }
+    for (int c = 0; c < ncj; ++c) { sb += part[((size_t)j * nch + c) * 2]; su += part[((size_t)j * nch + c) * 2 + 1]; }
     coef[j] = (float)(w2[j] * su - w1[j] * sb);
 }
 
-// u = x* + V coef
-__global__ void k_spec_apply(int n, int k, const float* __restrict__ V, const float* __restrict__ coef, const float* __restrict__ xs,
+// u = x* + V coef: bottom interface dof i of problem p takes x* from row
+// bot_pos[i] of the combined solution and the problem's modes [prob_m0[p], prob_m0[p+1])
+__global__ void k_spec_apply(int n, const int* __restrict__ row_prob, const int* __restrict__ prob_row0, const int* __restrict__ prob_n,
+                             const int* __restrict__ prob_m0, const long long* __restrict__ prob_vofs, const float* __restrict__ V,
+                             const float* __restrict__ coef, const float* __restrict__ sol, const int* __restrict__ bot_pos,
                              float* __restrict__ u) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float acc = xs[i];
-    for (int j = 0; j < k; ++j) acc += V[(size_t)j * n + i] * coef[j];
+    const int p = row_prob[i], r = i - prob_row0[p], np = prob_n[p], m0 = prob_m0[p], m1 = prob_m0[p + 1];
+    const float* Vp = V + prob_vofs[p];
+    float acc = sol[bot_pos[i]];
+    for (int j = m0; j < m1; ++j) acc += Vp[(size_t)(j - m0) * np + r] * coef[j];
     u[i] = acc;
+}
+
+// dst[pos[i]] = src[i] and its inverse, for the combined systems' interface rows
+__global__ void k_pos_scatter(int n, const int* __restrict__ pos, const float* __restrict__ src, float* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[pos[i]] = src[i];
+}
+__global__ void k_pos_gather(int n, const int* __restrict__ pos, const float* __restrict__ src, float* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[pos[i]];
 }
 
 }  // namespace
@@ -498,31 +524,33 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
 
 SigmaDevice::~SigmaDevice() {}
 
+static uint64_t csr_hash(const Csr& m) {
+    uint64_t h = 1469598103934665603ULL;
+    auto mix = [&](const void* p, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < bytes; ++i) { h ^= c[i]; h *= 1099511628211ULL; }
+    };
+    mix(&m.rows, sizeof(m.rows));
+    mix(m.ptr.data(), m.ptr.size() * sizeof(int));
+    mix(m.idx.data(), m.idx.size() * sizeof(int));
+    mix(m.val.data(), m.val.size() * sizeof(float));
+    return h;
+}
+static bool csr_equal(const Csr& a, const Csr& b) { return a.rows == b.rows && a.ptr == b.ptr && a.idx == b.idx && a.val == b.val; }
+
 // For each matrix, the index of the first matrix equal to it (same rows,
 // pattern and values), or its own index.
 static std::vector<int> identical_matrices(const std::vector<Csr>& A) {
-    auto hash = [](const Csr& m) {
-        uint64_t h = 1469598103934665603ULL;
-        auto mix = [&](const void* p, size_t bytes) {
-            const unsigned char* c = static_cast<const unsigned char*>(p);
-            for (size_t i = 0; i < bytes; ++i) { h ^= c[i]; h *= 1099511628211ULL; }
-        };
-        mix(&m.rows, sizeof(m.rows));
-        mix(m.ptr.data(), m.ptr.size() * sizeof(int));
-        mix(m.idx.data(), m.idx.size() * sizeof(int));
-        mix(m.val.data(), m.val.size() * sizeof(float));
-        return h;
-    };
     const int n = (int)A.size();
     std::vector<uint64_t> h(n);
 #pragma omp parallel for schedule(dynamic)
-    for (int s = 0; s < n; ++s) h[s] = hash(A[s]);
+    for (int s = 0; s < n; ++s) h[s] = csr_hash(A[s]);
     std::vector<int> rep(n);
     std::map<uint64_t, std::vector<int>> seen;
     for (int s = 0; s < n; ++s) {
         rep[s] = s;
         for (int c : seen[h[s]])
-            if (A[c].rows == A[s].rows && A[c].ptr == A[s].ptr && A[c].idx == A[s].idx && A[c].val == A[s].val) { rep[s] = c; break; }
+            if (csr_equal(A[c], A[s])) { rep[s] = c; break; }
         if (rep[s] == s) seen[h[s]].push_back(s);
     }
     return rep;
@@ -532,20 +560,49 @@ static std::vector<int> identical_matrices(const std::vector<Csr>& A) {
 static const double kBottomLambdaCut = std::pow(1e-9, 1.0 / 200.0);
 static const size_t kBottomBasisBytes = size_t(2) << 30;
 
-static void upload_spectrum(SigmaDevice& S, const BottomSpectrum& sp, int n, bool keep_sv) {
-    const int k = (int)sp.lambda.size();
-    S.bottom_k = k;
-    if (k == 0) return;
-    std::vector<float> v(sp.V.begin(), sp.V.end());
-    S.spec_V.upload(v);
-    if (keep_sv) { std::vector<float> sv(sp.SV.begin(), sp.SV.end()); S.spec_SV.upload(sv); }
-    std::vector<double> w1(k), w2(k);
-    for (int j = 0; j < k; ++j) { w2[j] = std::pow(sp.lambda[j], 200.0); w1[j] = w2[j] / (1.0 - sp.lambda[j]); }
-    S.spec_w1.upload(w1); S.spec_w2.upload(w2);
-    S.spec_nch = (n + kSpecChunk - 1) / kSpecChunk;
-    S.spec_part.alloc((size_t)2 * k * S.spec_nch);
-    S.spec_coef.alloc(k);
-    S.plan_bytes += S.spec_V.n * 4 + S.spec_SV.n * 4;
+static std::string bottom_mode() {
+    const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
+    const std::string m = mode ? mode : "auto";
+    if (m != "loop" && m != "direct" && m != "auto") throw std::runtime_error("HDD_SIGMA_BOTTOM must be loop, direct or auto");
+    return m;
+}
+
+void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
+    auto t0 = std::chrono::steady_clock::now();
+    F = SigmaFactors();
+    F.lv.resize(H.levels);
+    for (int l = 0; l < H.levels; ++l) {
+        SigmaFactors::Level& L = F.lv[l];
+        // subdomains of the same shape have identical Aii: factor each distinct matrix once
+        L.rep = identical_matrices(H.Aii[l]);
+        L.Aii.resize(H.nsub);
+        ParallelErrors errs;
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < H.nsub; ++s)
+            if (H.Aii[l][s].rows > 0 && L.rep[s] == s) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, L.Aii[s]); });
+        errs.rethrow();
+        std::vector<Int3> rc(H.if_coords[l].size());
+        for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l};
+        factor_spd(H.Arr[l], rc, 1e-5, L.Arr);
+        if (l == H.levels - 1 && H.n_if[l] > 0) {
+            const std::string m = bottom_mode();
+            if (m == "loop") continue;
+            Csr C; std::vector<Int3> xyz;
+            combined_bottom(H, l, C, xyz);
+            factor_spd(C, xyz, 1e-5, F.bottom);
+            F.bottom_n_int = C.rows - H.n_if[l];
+            if (m == "direct") {
+                F.bottom_direct = true;  // no low-rank correction: exact only when M^200 is negligible
+            } else {
+                bottom_spectrum(H.Arr[l], 1e-5, F.bottom, F.bottom_n_int, kBottomLambdaCut, kBottomBasisBytes, F.spec);
+                F.bottom_rho = F.spec.lambda_max; F.bottom_steps = F.spec.steps; F.bottom_seconds = F.spec.seconds;
+                F.bottom_direct = F.spec.resolved;
+                if (!F.spec.resolved) F.spec = BottomSpectrum();
+            }
+            if (!F.bottom_direct) F.bottom = SupernodalFactor();
+        }
+    }
+    F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Compares the bottom solve with the reference's 200 sweeps on one random
@@ -571,89 +628,141 @@ static double bottom_self_check(SigmaDevice& S) {
     return umax > 0 ? dmax / umax : dmax;
 }
 
-std::unique_ptr<SigmaDevice> make_sigma_device(const SigmaHost& H, int smoothing, int n_mg) {
+// rows of A appended to M with columns shifted by col_off
+static void append_rows(Csr& M, const Csr& A, int col_off) {
+    for (int r = 0; r < A.rows; ++r) {
+        for (int e = A.ptr[r]; e < A.ptr[r + 1]; ++e) { M.idx.push_back(A.idx[e] + col_off); M.val.push_back(A.val[e]); }
+        M.ptr.push_back((int)M.idx.size());
+    }
+    M.rows += A.rows;
+}
+
+std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost*>& hosts, const std::vector<const SigmaFactors*>& factors,
+                                               int smoothing, int n_mg) {
+    const int NP = (int)hosts.size();
+    if (NP == 0 || (int)factors.size() != NP) throw std::runtime_error("make_sigma_device: one host operator set and factor set per problem");
     std::unique_ptr<SigmaDevice> S(new SigmaDevice());
-    S->levels = H.levels; S->smoothing = smoothing; S->n_mg = n_mg;
-    S->lv.resize(H.levels);
+    const int NL = hosts[0]->levels;
+    for (int p = 0; p < NP; ++p)
+        if (hosts[p]->levels != NL || (int)factors[p]->lv.size() != NL)
+            throw std::runtime_error("make_sigma_device: every problem of a batch needs the same number of sigma levels");
+    S->levels = NL; S->smoothing = smoothing; S->n_mg = n_mg; S->nprob = NP;
+    S->lv.resize(NL);
     auto t0 = std::chrono::steady_clock::now();
-    for (int l = 0; l < H.levels; ++l) {
+    // identical interior matrices across the whole batch share one factor
+    std::map<uint64_t, std::vector<std::pair<const Csr*, const SupernodalFactor*>>> canon;
+    auto canonical = [&](const Csr& A, const SupernodalFactor* f) {
+        auto& bucket = canon[csr_hash(A)];
+        for (auto& e : bucket) if (csr_equal(*e.first, A)) return e.second;
+        bucket.push_back({&A, f});
+        return f;
+    };
+    std::vector<std::vector<int>> ifoff(NL + 1, std::vector<int>(NP + 1, 0));
+    for (int l = 0; l < NL; ++l)
+        for (int p = 0; p < NP; ++p) ifoff[l][p + 1] = ifoff[l][p] + hosts[p]->n_if[l];
+    for (int l = 0; l < NL; ++l) {
         SigmaLevel& L = S->lv[l];
-        L.n_if = H.n_if[l];
-        // concatenate the subdomains' interior systems
-        std::vector<int> off(H.nsub + 1, 0);
-        for (int s = 0; s < H.nsub; ++s) off[s + 1] = off[s] + H.Aii[l][s].rows;
-        L.n_int = off[H.nsub];
-        Csr air; air.rows = L.n_int; air.cols = L.n_if; air.ptr.assign(1, 0);
-        for (int s = 0; s < H.nsub; ++s) {
-            const Csr& A = H.Air[l][s];
-            for (int r = 0; r < A.rows; ++r) {
-                for (int e = A.ptr[r]; e < A.ptr[r + 1]; ++e) { air.idx.push_back(A.idx[e]); air.val.push_back(A.val[e]); }
-                air.ptr.push_back((int)air.idx.size());
+        L.n_if = ifoff[l][NP];
+        Csr air, arr, pr;
+        air.cols = L.n_if; arr.cols = L.n_if;
+        std::vector<const SupernodalFactor*> fi, fr;
+        std::vector<int> oi, orr;
+        int n_int = 0;
+        for (int p = 0; p < NP; ++p) {
+            const SigmaHost& H = *hosts[p];
+            const SigmaFactors::Level& F = factors[p]->lv[l];
+            for (int s = 0; s < H.nsub; ++s) {
+                append_rows(air, H.Air[l][s], ifoff[l][p]);
+                if (H.Aii[l][s].rows > 0) {
+                    const int rp = F.rep[s];
+                    fi.push_back(canonical(H.Aii[l][rp], &F.Aii[rp]));
+                    oi.push_back(n_int);
+                }
+                n_int += H.Aii[l][s].rows;
             }
+            append_rows(arr, H.Arr[l], ifoff[l][p]);
+            if (H.n_if[l] > 0) { fr.push_back(&F.Arr); orr.push_back(ifoff[l][p]); }
+            if (l + 1 < NL) append_rows(pr, H.Prr[l], ifoff[l + 1][p]);
         }
+        L.n_int = n_int;
+        std::map<const SupernodalFactor*, int> distinct;
+        for (auto f : fi) distinct[f] = 1;
+        L.n_distinct = (int)distinct.size();
         L.Air.upload(air);
         L.AirT.upload(transpose(air));
-        L.Arr.upload(H.Arr[l]);
-        if (l + 1 < H.levels) { L.P.upload(H.Prr[l]); L.PT.upload(transpose(H.Prr[l])); }
-        // subdomains of the same shape have identical Aii: factor each distinct matrix once
-        const std::vector<int> rep = identical_matrices(H.Aii[l]);
-        std::vector<SupernodalFactor> fi(H.nsub);
-        ParallelErrors errs;
-#pragma omp parallel for schedule(dynamic)
-        for (int s = 0; s < H.nsub; ++s)
-            if (H.Aii[l][s].rows > 0 && rep[s] == s) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, fi[s]); });
-        errs.rethrow();
-        std::vector<const SupernodalFactor*> fp;
-        std::vector<int> fo;
-        for (int s = 0; s < H.nsub; ++s) if (H.Aii[l][s].rows > 0) { fp.push_back(&fi[rep[s]]); fo.push_back(off[s]); }
-        L.Lii.build(fp, fo);
-        L.n_distinct = 0;
-        for (int s = 0; s < H.nsub; ++s) L.n_distinct += H.Aii[l][s].rows > 0 && rep[s] == s;
-        std::vector<Int3> rc(H.if_coords[l].size());
-        for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l};
-        SupernodalFactor fr;
-        factor_spd(H.Arr[l], rc, 1e-5, fr);
-        L.Lrr.build({&fr}, {0});
-        if (l == H.levels - 1 && L.n_if > 0) {
-            const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
-            const std::string m = mode ? mode : "auto";
-            if (m != "loop" && m != "direct" && m != "auto") throw std::runtime_error("HDD_SIGMA_BOTTOM must be loop, direct or auto");
-            if (m != "loop") {
-                Csr C; std::vector<Int3> xyz;
-                combined_bottom(H, l, C, xyz);
-                SupernodalFactor fc;
-                factor_spd(C, xyz, 1e-5, fc);
-                if (m == "direct") {
-                    S->bottom_direct = true;  // no low-rank correction: exact only when M^200 is negligible
-                } else {
-                    BottomSpectrum sp;
-                    bottom_spectrum(H.Arr[l], 1e-5, fc, L.n_int, kBottomLambdaCut, kBottomBasisBytes, sp);
-                    S->bottom_rho = sp.lambda_max; S->bottom_steps = sp.steps; S->bottom_seconds = sp.seconds;
-                    if (sp.resolved) {
-                        S->bottom_direct = true;
-                        upload_spectrum(*S, sp, L.n_if, H.levels == 1);
-                    }
-                }
-                if (S->bottom_direct) {
-                    S->Lbottom.build({&fc}, {0});
-                    S->bot_rhs.alloc(C.rows); S->bot_sol.alloc(C.rows);
-                    S->plan_bytes += S->Lbottom.bytes();
-                }
-            }
-        }
-        std::vector<SupernodalFactor>().swap(fi);
+        L.Arr.upload(arr);
+        if (l + 1 < NL) { pr.cols = ifoff[l + 1][NP]; L.P.upload(pr); L.PT.upload(transpose(pr)); }
+        L.Lii.build(fi, oi);
+        L.Lrr.build(fr, orr);
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
         S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
     }
+    // bottom level: one block of the combined plan and one set of modes per problem
+    const int lb = NL - 1;
+    bool any_if = false, all_direct = true;
+    for (int p = 0; p < NP; ++p) {
+        const SigmaFactors& F = *factors[p];
+        S->bottom_rho = std::max(S->bottom_rho, F.bottom_rho);
+        S->bottom_steps = std::max(S->bottom_steps, F.bottom_steps);
+        S->bottom_seconds += F.bottom_seconds;
+        if (hosts[p]->n_if[lb] == 0) continue;
+        any_if = true;
+        all_direct = all_direct && F.bottom_direct;
+    }
+    if (NL > 0 && any_if && all_direct) {
+        const int n = ifoff[lb][NP];
+        std::vector<const SupernodalFactor*> fb;
+        std::vector<int> ob, pos(n), rprob(n), m0(NP + 1, 0), prow0(NP), pn(NP), mrow0, mlen;
+        std::vector<long long> pvofs(NP, 0), mvofs;
+        std::vector<float> V, SV;
+        std::vector<double> w1, w2;
+        const bool keep_sv = NL == 1;
+        int boff = 0;
+        for (int p = 0; p < NP; ++p) {
+            const SigmaFactors& F = *factors[p];
+            const int np = hosts[p]->n_if[lb], r0 = ifoff[lb][p];
+            prow0[p] = r0; pn[p] = np; pvofs[p] = (long long)V.size();
+            const int k = np > 0 ? (int)F.spec.lambda.size() : 0;
+            m0[p + 1] = m0[p] + k;
+            if (np == 0) continue;
+            fb.push_back(&F.bottom); ob.push_back(boff);
+            for (int i = 0; i < np; ++i) { pos[r0 + i] = boff + F.bottom_n_int + i; rprob[r0 + i] = p; }
+            for (int j = 0; j < k; ++j) {
+                mrow0.push_back(r0); mlen.push_back(np); mvofs.push_back((long long)V.size() + (long long)j * np);
+                const double lam = F.spec.lambda[j];
+                w2.push_back(std::pow(lam, 200.0)); w1.push_back(std::pow(lam, 200.0) / (1.0 - lam));
+            }
+            V.insert(V.end(), F.spec.V.begin(), F.spec.V.begin() + (size_t)np * k);
+            if (keep_sv) SV.insert(SV.end(), F.spec.SV.begin(), F.spec.SV.begin() + (size_t)np * k);
+            S->spec_nch = std::max(S->spec_nch, (np + kSpecChunk - 1) / kSpecChunk);
+            boff += F.bottom_n_int + np;
+        }
+        S->Lbottom.build(fb, ob);
+        S->bot_rhs.alloc(boff); S->bot_sol.alloc(boff);
+        S->bot_pos.upload(pos);
+        S->bottom_k = m0[NP];
+        if (S->bottom_k > 0) {
+            S->row_prob.upload(rprob); S->prob_m0.upload(m0); S->prob_row0.upload(prow0); S->prob_n.upload(pn); S->prob_vofs.upload(pvofs);
+            S->mode_row0.upload(mrow0); S->mode_len.upload(mlen); S->mode_vofs.upload(mvofs);
+            S->spec_V.upload(V);
+            if (keep_sv) S->spec_SV.upload(SV);
+            S->spec_w1.upload(w1); S->spec_w2.upload(w2);
+            S->spec_part.alloc((size_t)2 * S->bottom_k * S->spec_nch);
+            S->spec_coef.alloc(S->bottom_k);
+        }
+        S->bottom_direct = true;
+        S->plan_bytes += S->Lbottom.bytes() + S->spec_V.n * 4 + S->spec_SV.n * 4;
+    }
     if (S->bottom_direct) {
         S->bottom_check = bottom_self_check(*S);
-        const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
         // the reference's sweeps stay in charge if the shortcut disagrees with them
-        if (!(mode && std::string(mode) == "direct") && !(S->bottom_check < 1e-3)) S->bottom_direct = false;
+        if (bottom_mode() != "direct" && !(S->bottom_check < 1e-3)) S->bottom_direct = false;
     }
-    S->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (H.levels > 0) { S->b0 = S->lv[0].b.p; S->u0 = S->lv[0].u.p; }
+    double fs = 0;
+    for (int p = 0; p < NP; ++p) fs += factors[p]->seconds;
+    S->factor_seconds = fs + std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (NL > 0) { S->b0 = S->lv[0].b.p; S->u0 = S->lv[0].u.p; }
     return S;
 }
 
@@ -682,21 +791,26 @@ void SigmaDevice::residual(int l, hipStream_t s) {
                        (const float*)L.vx.p, (const int*)L.Arr.p.p, (const int*)L.Arr.ix.p, (const float*)L.Arr.v.p, (const float*)L.u.p, L.r.p);
 }
 
-// x* from the combined system [[Aii+eI, Air],[Air^T, Arr+eI]] [y; x*] = [0; b]
-// (the interior part of bot_rhs stays zero), then u = x* + V coef.
+// x* from the combined systems [[Aii+eI, Air],[Air^T, Arr+eI]] [y; x*] = [0; b]
+// (one block per problem; the interior rows of bot_rhs stay zero), then u = x* + V coef.
 void SigmaDevice::bottom(hipStream_t s, bool u_zero) {
     SigmaLevel& B = lv[levels - 1];
-    const int ni = B.n_int, n = B.n_if;
+    const int n = B.n_if;
     if (!u_zero && bottom_k && !spec_SV.p) throw std::runtime_error("bottom solve from a nonzero start needs S V");
-    launch_copy(bot_rhs.p + ni, B.b.p, n, s);
+    hipLaunchKernelGGL(k_pos_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const int*)bot_pos.p, (const float*)B.b.p, bot_rhs.p);
     Lbottom.solve(bot_rhs.p, bot_sol.p, s);
-    if (bottom_k == 0) { launch_copy(B.u.p, bot_sol.p + ni, n, s); return; }
-    hipLaunchKernelGGL(k_spec_dot, dim3(spec_nch, bottom_k), dim3(256), 0, s, n, spec_nch, (const float*)spec_V.p,
-                       (const float*)spec_SV.p, (const float*)B.b.p, u_zero ? (const float*)nullptr : (const float*)B.u.p, spec_part.p);
-    hipLaunchKernelGGL(k_spec_coef, dim3(blocks_for(bottom_k, 64)), dim3(64), 0, s, bottom_k, spec_nch, (const double*)spec_part.p,
-                       (const double*)spec_w1.p, (const double*)spec_w2.p, spec_coef.p);
-    hipLaunchKernelGGL(k_spec_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, bottom_k, (const float*)spec_V.p, (const float*)spec_coef.p,
-                       (const float*)(bot_sol.p + ni), B.u.p);
+    if (bottom_k == 0) {
+        hipLaunchKernelGGL(k_pos_gather, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const int*)bot_pos.p, (const float*)bot_sol.p, B.u.p);
+        return;
+    }
+    hipLaunchKernelGGL(k_spec_dot, dim3(spec_nch, bottom_k), dim3(256), 0, s, spec_nch, (const int*)mode_row0.p, (const int*)mode_len.p,
+                       (const long long*)mode_vofs.p, (const float*)spec_V.p, (const float*)spec_SV.p, (const float*)B.b.p,
+                       u_zero ? (const float*)nullptr : (const float*)B.u.p, spec_part.p);
+    hipLaunchKernelGGL(k_spec_coef, dim3(blocks_for(bottom_k, 64)), dim3(64), 0, s, bottom_k, spec_nch, (const int*)mode_len.p,
+                       (const double*)spec_part.p, (const double*)spec_w1.p, (const double*)spec_w2.p, spec_coef.p);
+    hipLaunchKernelGGL(k_spec_apply, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const int*)row_prob.p, (const int*)prob_row0.p,
+                       (const int*)prob_n.p, (const int*)prob_m0.p, (const long long*)prob_vofs.p, (const float*)spec_V.p,
+                       (const float*)spec_coef.p, (const float*)bot_sol.p, (const int*)bot_pos.p, B.u.p);
 }
 
 void SigmaDevice::vcycle(hipStream_t s, bool first) {

@@ -43,7 +43,8 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
            "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats",
            "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats",
-           "hdd_profile_solve"]
+           "hdd_profile_solve", "hdd_create_batch", "hdd_batch_count", "hdd_batch_num_pages", "hdd_batch_page_offsets",
+           "hdd_batch_pcg_solve", "hdd_batch_get_x"]
 
 
 def lib():
@@ -82,6 +83,12 @@ def lib():
         L.hdd_profile_solve.argtypes = [P, I, I, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hdd_stats.argtypes = [P, P, I]
         L.hdd_bottom_spectrum.argtypes = [P, C.c_double, P, I, C.POINTER(I), C.POINTER(C.c_double)]
+        L.hdd_create_batch.argtypes = [C.POINTER(ProblemDesc), I, C.POINTER(C.c_void_p)]
+        L.hdd_batch_count.argtypes = [P, C.POINTER(I)]
+        L.hdd_batch_num_pages.argtypes = [P, I, C.POINTER(I)]
+        L.hdd_batch_page_offsets.argtypes = [P, I, P]
+        L.hdd_batch_pcg_solve.argtypes = [P, C.POINTER(PcgParams), C.POINTER(PcgReport), P, I]
+        L.hdd_batch_get_x.argtypes = [P, I, P]
         for name in EXPORTS:
             if name != "hdd_last_error":
                 getattr(L, name).restype = I
@@ -93,23 +100,28 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _desc(dom, cfg):
+    keep = [np.ascontiguousarray(dom.page_offsets, dtype=np.uint64),
+            np.ascontiguousarray(dom.flags, dtype=np.uint32),
+            np.ascontiguousarray(dom.b, dtype=np.float32)]
+    d = ProblemDesc()
+    for a in range(3):
+        d.size[a] = int(dom.size[a])
+        d.sub_size[a] = int(cfg.sub_size)
+    d.vlevels, d.sigma_levels, d.nVcycles = cfg.vlevels, cfg.sigma_levels, cfg.nVcycles
+    d.interior_smoothing, d.boundary_smoothing = cfg.interior_smoothing, cfg.boundary_smoothing
+    d.sigma_smoothing, d.sigma_mg = cfg.sigma_smoothing, cfg.sigma_mg
+    d.n_pages = keep[0].shape[0]
+    d.page_offsets, d.flags, d.b = (_ptr(a) for a in keep)
+    return d, keep
+
+
 class DDSolver:
     """Host facade over one hdd_ctx (Read_SPGrid + Initialize_Preconditioner in the constructor)."""
 
     def __init__(self, dom, cfg, host_only: bool = False):
         L = lib()
-        self._keep = [np.ascontiguousarray(dom.page_offsets, dtype=np.uint64),
-                      np.ascontiguousarray(dom.flags, dtype=np.uint32),
-                      np.ascontiguousarray(dom.b, dtype=np.float32)]
-        d = ProblemDesc()
-        for a in range(3):
-            d.size[a] = int(dom.size[a])
-            d.sub_size[a] = int(cfg.sub_size)
-        d.vlevels, d.sigma_levels, d.nVcycles = cfg.vlevels, cfg.sigma_levels, cfg.nVcycles
-        d.interior_smoothing, d.boundary_smoothing = cfg.interior_smoothing, cfg.boundary_smoothing
-        d.sigma_smoothing, d.sigma_mg = cfg.sigma_smoothing, cfg.sigma_mg
-        d.n_pages = self._keep[0].shape[0]
-        d.page_offsets, d.flags, d.b = (_ptr(a) for a in self._keep)
+        d, self._keep = _desc(dom, cfg)
         h = C.c_void_p()
         st = (L.hdd_create_host_only if host_only else L.hdd_create)(C.byref(d), C.byref(h))
         if st != 0:
@@ -262,3 +274,52 @@ class DDSolver:
 
     # reference-facade spellings
     Solve = solve
+
+
+class DDBatch(DDSolver):
+    """Several independent problems in one context (hdd_create_batch): one launch per stage for all of them."""
+
+    def __init__(self, problems):
+        L = lib()
+        descs = (ProblemDesc * len(problems))()
+        self._keep = []
+        for i, (dom, cfg) in enumerate(problems):
+            d, keep = _desc(dom, cfg)
+            descs[i] = d
+            self._keep += keep
+        h = C.c_void_p()
+        st = L.hdd_create_batch(descs, len(problems), C.byref(h))
+        if st != 0:
+            raise HddError(f"hdd_create_batch failed ({st}): {L.hdd_last_error(None).decode()}")
+        self.h = h
+        n = C.c_int()
+        self._check(L.hdd_num_pages(self.h, C.byref(n)))
+        self.n_pages = n.value
+        self._check(L.hdd_batch_count(self.h, C.byref(n)))
+        self.count = n.value
+
+    def problem_pages(self, p):
+        n = C.c_int()
+        self._check(lib().hdd_batch_num_pages(self.h, p, C.byref(n)))
+        return n.value
+
+    def problem_page_offsets(self, p):
+        out = np.zeros(self.problem_pages(p), dtype=np.uint64)
+        self._check(lib().hdd_batch_page_offsets(self.h, p, _ptr(out)))
+        return out
+
+    def solve_all(self, tolerances, max_iterations: int = 5000):
+        """PCG from x=0 for every problem; returns a list of (iterations, residual history, seconds)."""
+        n = self.count
+        tol = np.broadcast_to(np.asarray(tolerances, dtype=np.float64), (n,))
+        prm = (PcgParams * n)(*[PcgParams(float(t), int(max_iterations)) for t in tol])
+        rep = (PcgReport * n)()
+        hl = max_iterations + 1
+        hist = np.zeros((n, hl), dtype=np.float64)
+        self._check(lib().hdd_batch_pcg_solve(self.h, prm, rep, _ptr(hist), hl), ok=(0, 4))
+        return [(rep[p].iterations, hist[p, :abs(rep[p].iterations) + 1].copy(), rep[p].seconds) for p in range(n)]
+
+    def x(self, p):
+        out = np.zeros((self.problem_pages(p), 256), dtype=np.float32)
+        self._check(lib().hdd_batch_get_x(self.h, p, _ptr(out)))
+        return out

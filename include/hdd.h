@@ -71,6 +71,39 @@ hdd_status hdd_create(const hdd_problem_desc* desc, hdd_ctx** out);
  * flags and interface operators on machines without a GPU. */
 hdd_status hdd_create_host_only(const hdd_problem_desc* desc, hdd_ctx** out);
 
+/* Batch of independent problems in one context (no reference counterpart:
+ * the reference solves one problem per process). Each problem is set up as
+ * by hdd_create; their pages share one device grid and their interface
+ * operators one block-diagonal interface solver, so every V-cycle stage,
+ * triangular-solve height and PCG vector operation is one launch for all of
+ * them. Every problem needs the same vlevels, sigma_levels and smoothing
+ * counts; sizes, sub_size and geometry may differ. Each problem keeps its own
+ * PCG scalars and stopping test (hdd_batch_pcg_solve). The single-problem
+ * entry points act on the combined grid (channels, level flags, statistics);
+ * interface-operator introspection refers to problem 0. */
+hdd_status hdd_create_batch(const hdd_problem_desc* descs, int count, hdd_ctx** out);
+
+/* Number of problems in a context (1 for hdd_create). */
+hdd_status hdd_batch_count(const hdd_ctx* ctx, int* count);
+
+/* Level-0 pages of one problem and their SPGrid offsets (Morton order,
+ * the problem's own coordinates). */
+hdd_status hdd_batch_num_pages(const hdd_ctx* ctx, int problem, int* n_pages);
+hdd_status hdd_batch_page_offsets(const hdd_ctx* ctx, int problem, uint64_t* out);
+
+/* PCG from x = 0 for every problem at once: params[p], reports[p] per
+ * problem; residual_history (may be NULL) receives max|r| per iteration,
+ * history_len values per problem (row p at p*history_len, zero-padded).
+ * Per problem the iteration is CONJUGATE_GRADIENT_DUMPING::Solve
+ * (DDS/CONJUGATE_GRADIENT_DUMPING.cpp:146-212); a problem that has met its
+ * tolerance keeps its x while the others continue. Returns
+ * HDD_ERR_NOT_CONVERGED if any problem stopped at its iteration limit. */
+hdd_status hdd_batch_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* params, hdd_pcg_report* reports,
+                               double* residual_history, int history_len);
+
+/* x of one problem: n_pages(problem)*256 floats in the problem's page order. */
+hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host);
+
 /* Releases all device and host memory of a context. */
 hdd_status hdd_destroy(hdd_ctx* ctx);
 
