@@ -12,6 +12,20 @@ object covers the level-0 damped-Jacobi sweep kernel, timed with HIP events
 on the context stream; cpu_baseline is the oracle (the CPU restatement of the
 reference dd_pcg) on a bounded sample: the complete 64^3 problem (setup +
 solve), set beside the GPU's rate on the same 64^3 problem.
+
+The other two parts of the metric are added to the same line:
+  "batched": uneven independent problems (edge 64-256, 1-4 sphere obstacles,
+      delta right-hand side, sub_size 32; --batch-count of them, 16 by default,
+      64 for the full set) solved one at a time on the GPU and then all at once
+      in one batched context; solves/s of both (solve phase only: host setup is
+      the same for both and is reported separately) and each batched result's
+      difference from its solve alone;
+  "served": a request stream (edge log-uniform 32-384, up to 40% blocked by
+      spheres, N(0,1) right-hand side, tolerance 1e-6) with Poisson arrivals at
+      the rates of --serve-rates, through the serving loop (host setup workers,
+      groups of batched solves side by side on the GPU); p50/p99 latency
+      (submit to result, host setup included) and solves/s per rate, beside the
+      plain path (each request set up and solved alone, one after another).
 """
 from __future__ import annotations
 
@@ -43,6 +57,11 @@ def main():
     ap.add_argument("--config", default="sphere256", choices=["sphere256", "sphere64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=500)
+    ap.add_argument("--batch-count", type=int, default=16, help="problems of the batched leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=8, help="requests per rate of the served leg (0: skip)")
+    ap.add_argument("--serve-rates", default="0.5,2,8", help="comma-separated Poisson arrival rates (requests/s) of the served leg")
+    ap.add_argument("--serve-nmax", type=int, default=256,
+                    help="largest request edge of the served leg (384 for the full stream; 256 keeps the default run short)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="set up, then run only level-0 sweeps and a plain copy of known size (for counter runs)")
     args = ap.parse_args()
@@ -88,9 +107,13 @@ def main():
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic, {achieved:.0f} GB/s")
 
+    solver.close()
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(hdd)
+    batched = batched_leg(hdd, args.batch_count) if args.batch_count > 0 else None
+    served = served_leg(hdd, args.serve_requests, [float(r) for r in args.serve_rates.split(",")], args.serve_nmax) \
+        if args.serve_requests > 0 else None
 
     out = {
         "metric": METRIC,
@@ -113,6 +136,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": counter_traffic(dom.n_pages) if args.config == "sphere256" else None},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 
@@ -147,6 +172,93 @@ def counter_traffic(pages):
     if sweep_grid < pages * 256:
         return None
     return 1024.0 * (fetch["k_smooth"][sweep_grid] * f_scale + write["k_smooth"][sweep_grid] * w_scale)
+
+
+def batched_leg(hdd, count, seed=0):
+    """Uneven problems one at a time on the GPU, then all of them in one batched context."""
+    from hierarchicaldd_amd import domain
+    t0 = time.time()
+    probs = domain.batch_problems(count, seed=seed)
+    log(f"[batched] {count} problems generated in {time.time() - t0:.1f}s "
+        f"(edges {sorted(d.meta['n'] for d, _ in probs)}, {sum(d.n_pages for d, _ in probs)} pages)")
+    singles, plain_s, setup_s, iters, xs = [], 0.0, 0.0, [], []
+    for dom, cfg in probs:
+        g = hdd.DDSolver(dom, cfg)
+        setup_s += g.setup_seconds
+        it, hist, secs = g.solve(1e-6 * float(np.abs(dom.b).max()), 1000)
+        plain_s += secs
+        iters.append(it)
+        xs.append(g.get(hdd.CH_X))
+        singles.append(g)
+    log(f"[batched] plain: {count} solves in {plain_s:.3f}s ({count / plain_s:.2f} solves/s), host setup {setup_s:.1f}s")
+    t0 = time.time()
+    bt = hdd.DDBatch(from_solvers=singles)
+    build_s = time.time() - t0
+    for g in singles:
+        g.close()
+    tol = [1e-6 * float(np.abs(d.b).max()) for d, _ in probs]
+    bt.solve_all(tol, 1000)  # warm-up
+    t0 = time.perf_counter()
+    res = bt.solve_all(tol, 1000)
+    batch_s = time.perf_counter() - t0
+    bit = [r[0] for r in res]
+    xdiff = max(float(np.abs(bt.x(p) - xs[p]).max() / max(np.abs(xs[p]).max(), 1e-30)) for p in range(count))
+    bt.close()
+    log(f"[batched] batch: {count} solves in {batch_s:.3f}s ({count / batch_s:.2f} solves/s, x {plain_s / batch_s:.2f} the plain path), "
+        f"device build {build_s:.1f}s, iterations alone {iters} batched {bit}, max x rel diff {xdiff:.2e}")
+    return {"problems": count, "seed": seed, "plain_solves_per_s": count / plain_s, "batched_solves_per_s": count / batch_s,
+            "speedup": plain_s / batch_s, "plain_solve_s": plain_s, "batched_solve_s": batch_s, "host_setup_s": setup_s,
+            "batch_device_build_s": build_s, "max_iteration_diff": int(max(abs(a - b) for a, b in zip(iters, bit))),
+            "max_x_rel_diff_vs_alone": xdiff, "all_converged": bool(min(bit) >= 0)}
+
+
+def served_leg(hdd, n_req, rates, nmax, seed=1):
+    """Poisson request stream through the serving loop, and the same requests set up and solved one by one."""
+    from hierarchicaldd_amd import domain
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    reqs = [domain.served_request(rng, nmax=nmax) for _ in range(n_req)]
+    log(f"[served] {n_req} requests generated in {time.time() - t0:.1f}s (edges {[d.meta['n'] for d, _ in reqs]})")
+    tols = [1e-6 * float(np.abs(d.b).max()) for d, _ in reqs]
+    # plain path: each request set up and solved alone, one after another
+    t0 = time.perf_counter()
+    plain_lat = []
+    for k, ((dom, cfg), tol) in enumerate(zip(reqs, tols)):
+        t1 = time.perf_counter()
+        g = hdd.DDSolver(dom, cfg)
+        it, _, secs = g.solve(tol, 1000)
+        g.close()
+        plain_lat.append(time.perf_counter() - t1)
+        log(f"[served] plain request {k} (edge {dom.meta['n']}): {it} iterations, solve {secs:.3f}s, with setup {plain_lat[-1]:.2f}s")
+    plain_s = time.perf_counter() - t0
+    log(f"[served] plain: {n_req} requests in {plain_s:.1f}s ({n_req / plain_s:.3f} solves/s)")
+    out = {"requests": n_req, "seed": seed, "edge_max": nmax, "plain_solves_per_s": n_req / plain_s,
+           "plain_p50_s": float(np.percentile(plain_lat, 50)), "rates": []}
+    for lam in rates:
+        arr = np.cumsum(rng.exponential(1.0 / lam, size=n_req))
+        srv = hdd.Server(setup_threads=2, groups=2, max_batch=16)
+        t0 = time.perf_counter()
+        ids = []
+        for (dom, cfg), tol, ta in zip(reqs, tols, arr):
+            dt = ta - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(dom, cfg, tol, 1000))
+        reps = []
+        for k, i in enumerate(ids):
+            reps.append(srv.wait(i)[2])
+            log(f"[served] rate {lam}/s request {k}: latency {reps[-1]['latency_s']:.2f}s (group of {reps[-1]['batch_size']})")
+        span = time.perf_counter() - t0
+        srv.close()
+        lat = [r["latency_s"] for r in reps]
+        ok = all(r["iterations"] >= 0 for r in reps)
+        row = {"rate_per_s": lam, "solves_per_s": n_req / span, "p50_latency_s": float(np.percentile(lat, 50)),
+               "p99_latency_s": float(np.percentile(lat, 99)), "mean_setup_s": float(np.mean([r["setup_s"] for r in reps])),
+               "mean_queue_s": float(np.mean([r["queue_s"] for r in reps])), "mean_solve_s": float(np.mean([r["solve_s"] for r in reps])),
+               "max_group": int(max(r["batch_size"] for r in reps)), "all_converged": ok}
+        log(f"[served] rate {lam}/s: {row}")
+        out["rates"].append(row)
+    return out
 
 
 def cpu_baseline(hdd):

@@ -1,12 +1,14 @@
 // extern "C" boundary of the library (declarations and reference citations in include/hdd.h).
 #include "../../include/hdd.h"
 #include "hdd_runtime.h"
+#include "hdd_server.h"
 #include "hdd_sigma_device.h"
 #include "hdd_spectral.h"
 #include "hdd_triplan.h"
 #include "hdd_parallel.h"
 #include "hdd_factor.h"
 #include <cstring>
+#include <mutex>
 #include <string>
 
 using namespace hdd;
@@ -91,6 +93,26 @@ hdd_status hdd_create_batch(const hdd_problem_desc* descs, int count, hdd_ctx** 
     }
 }
 
+hdd_status hdd_create_batch_from(hdd_ctx* const* ctxs, int count, hdd_ctx** out) {
+    if (!ctxs || !out || count < 1) { g_create_error = "hdd_create_batch_from: invalid arguments"; return HDD_ERR_ARG; }
+    try {
+        std::vector<std::shared_ptr<ProblemHost>> probs;
+        for (int i = 0; i < count; ++i) {
+            if (!ctxs[i]) throw std::runtime_error("hdd_create_batch_from: null context");
+            for (auto& q : ctxs[i]->solver->prob) probs.push_back(q);
+        }
+        auto* c = new hdd_ctx();
+        try {
+            c->solver.reset(new Solver(probs));
+        } catch (...) { delete c; throw; }
+        *out = c;
+        return HDD_OK;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return HDD_ERR_SETUP;
+    }
+}
+
 hdd_status hdd_batch_count(const hdd_ctx* ctx, int* count) {
     if (!ctx || !count) return HDD_ERR_ARG;
     *count = ctx->solver->nprob();
@@ -142,6 +164,12 @@ hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host) {
     HDD_TRY(ctx, ctx->solver->download_x(problem, host));
 }
 
+hdd_status hdd_set_graph(hdd_ctx* ctx, int enable) {
+    if (!ctx) return HDD_ERR_ARG;
+    ctx->solver->use_graph = enable != 0;
+    return HDD_OK;
+}
+
 hdd_status hdd_destroy(hdd_ctx* ctx) {
     if (!ctx) return HDD_ERR_ARG;
     delete ctx;
@@ -169,7 +197,8 @@ hdd_status hdd_set_channel(hdd_ctx* ctx, int ch, const float* host) {
     HDD_TRY(ctx, {
         Solver& S = *ctx->solver;
         const size_t n = (size_t)S.lv[0].host.npages() * kPageCells;
-        hdd_check(hipMemcpy(S.channel(ch), host, n * sizeof(float), hipMemcpyHostToDevice), "set_channel");
+        hdd_check(hipMemcpyAsync(S.channel(ch), host, n * sizeof(float), hipMemcpyHostToDevice, S.stream), "set_channel");
+        hdd_check(hipStreamSynchronize(S.stream), "set_channel");
     })
 }
 
@@ -180,7 +209,8 @@ hdd_status hdd_get_channel(hdd_ctx* ctx, int ch, float* host) {
         Solver& S = *ctx->solver;
         hdd_check(hipStreamSynchronize(S.stream), "sync");
         const size_t n = (size_t)S.lv[0].host.npages() * kPageCells;
-        hdd_check(hipMemcpy(host, S.channel(ch), n * sizeof(float), hipMemcpyDeviceToHost), "get_channel");
+        hdd_check(hipMemcpyAsync(host, S.channel(ch), n * sizeof(float), hipMemcpyDeviceToHost, S.stream), "get_channel");
+        hdd_check(hipStreamSynchronize(S.stream), "get_channel");
     })
 }
 
@@ -442,6 +472,92 @@ hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas,
         for (int j = 0; j < *k && j < cap; ++j) lambdas[j] = sp.lambda[j];
         if (lambda_max) *lambda_max = sp.lambda_max;
     })
+}
+
+// ---- serving
+
+struct hdd_server {
+    std::unique_ptr<Server> srv;
+    std::mutex mu;
+    std::string err;
+};
+
+hdd_status hdd_server_create(const hdd_server_params* p, hdd_server** out) {
+    if (!p || !out || p->setup_threads < 1 || p->groups < 1 || p->max_batch < 1 || p->hbm_budget_bytes < 0) {
+        g_create_error = "hdd_server_create: invalid arguments";
+        return HDD_ERR_ARG;
+    }
+    try {
+        ServerParams sp;
+        sp.setup_threads = p->setup_threads; sp.groups = p->groups; sp.max_batch = p->max_batch; sp.hbm_budget = p->hbm_budget_bytes;
+        auto* s = new hdd_server();
+        try {
+            s->srv.reset(new Server(sp));
+        } catch (...) { delete s; throw; }
+        *out = s;
+        return HDD_OK;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return HDD_ERR_DEVICE;
+    }
+}
+
+hdd_status hdd_server_submit(hdd_server* s, const hdd_problem_desc* d, const hdd_pcg_params* prm, int64_t* id) {
+    if (!s || !prm || !id || prm->max_iterations < 0) return HDD_ERR_ARG;
+    std::string why;
+    if (!desc_ok(d, why)) { std::lock_guard<std::mutex> lk(s->mu); s->err = "hdd_server_submit: " + why; return HDD_ERR_ARG; }
+    try {
+        *id = s->srv->submit(params_of(d), d->n_pages, d->page_offsets, d->flags, d->b, prm->tolerance, prm->max_iterations);
+        return HDD_OK;
+    } catch (std::exception& e) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = e.what();
+        return HDD_ERR_ARG;
+    }
+}
+
+hdd_status hdd_server_wait(hdd_server* s, int64_t id, float* x, hdd_request_report* rep) {
+    if (!s) return HDD_ERR_ARG;
+    try {
+        std::shared_ptr<ServerRequest> r = s->srv->wait(id);
+        s->srv->forget(id);
+        if (rep) {
+            rep->iterations = r->iterations; rep->final_norm = r->final_norm;
+            rep->latency_s = r->t_done - r->t_submit;
+            rep->setup_s = r->t_ready - r->t_submit;
+            rep->queue_s = r->failed ? 0.0 : r->t_admit - r->t_ready;
+            rep->solve_s = r->failed ? 0.0 : r->t_done - r->t_admit;
+            rep->batch_size = r->batch_size;
+        }
+        if (r->failed) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->err = r->error;
+            return r->t_admit > 0 ? HDD_ERR_DEVICE : HDD_ERR_SETUP;
+        }
+        if (x) std::memcpy(x, r->x.data(), r->x.size() * sizeof(float));
+        return r->iterations < 0 ? HDD_ERR_NOT_CONVERGED : HDD_OK;
+    } catch (std::exception& e) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = e.what();
+        return HDD_ERR_ARG;
+    }
+}
+
+hdd_status hdd_server_destroy(hdd_server* s) {
+    if (!s) return HDD_ERR_ARG;
+    delete s;
+    return HDD_OK;
+}
+
+const char* hdd_server_last_error(const hdd_server* s) {
+    if (!s) return g_create_error.c_str();
+    return s->err.c_str();
+}
+
+hdd_status hdd_server_budget(const hdd_server* s, double* bytes) {
+    if (!s || !bytes) return HDD_ERR_ARG;
+    *bytes = s->srv->budget();
+    return HDD_OK;
 }
 
 }  // extern "C"

@@ -15,6 +15,8 @@ inline void hdd_check(hipError_t e, const char* what = "HIP call") {
 // One paged level on the device. flags has (np+1)*256 entries (zero page last),
 // nbr np*6 face-neighbour pages (np = zero page), base np*3 cell origins,
 // lut the block lookup table over nbx*nby*nbz blocks (-1 = unallocated).
+// A stopped problem's pages keep stale values: no kernel reads them for another
+// problem (no stencil crosses problems), and the problem's result is final.
 struct DevLevel {
     int np = 0;
     int nbx = 0, nby = 0, nbz = 0;
@@ -22,6 +24,10 @@ struct DevLevel {
     const int* nbr = nullptr;
     const int* base = nullptr;
     const int* lut = nullptr;
+    // batch: problem of each page, and per-problem flags (1 = iterating); pages of
+    // stopped problems are skipped by the stencil kernels. active == nullptr: none skipped.
+    const int* prob = nullptr;
+    const int* active = nullptr;
 };
 
 void upload_constants();

@@ -17,6 +17,7 @@
 //   PCG Laplace (double accumulation)       DDS/SPGrid_Gemini_Arithmetic_Helpers.h:240-262
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 #include "hdd_device.h"
 
 namespace hdd {
@@ -35,6 +36,8 @@ __host__ __device__ inline int xcd_chunk(int np) { return (np + kXcds - 1) / kXc
 __device__ __forceinline__ int xcd_page(int np) {
     return (int)(blockIdx.x % kXcds) * xcd_chunk(np) + (int)(blockIdx.x / kXcds);
 }
+
+__device__ __forceinline__ bool page_stopped(const DevLevel& L, int p) { return L.active && !L.active[L.prob[p]]; }
 
 __device__ __forceinline__ int nb_index(const int* __restrict__ nbr, int p, int c, int f) {
     switch (f) {
@@ -69,7 +72,7 @@ __device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int 
 __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restrict__ u_in, float* __restrict__ u_out,
                                                 const float* __restrict__ b, int write_interior, int write_boundary) {
     const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= L.np) return;
+    if (p >= L.np || page_stopped(L, p)) return;
     const uint32_t f = L.flags[e];
     const uint32_t sel = f & (kActive | kBoundary);
     const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
@@ -82,7 +85,7 @@ __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restr
 __global__ void __launch_bounds__(256) k_residual(DevLevel L, const float* __restrict__ u, const float* __restrict__ b,
                                                   float* __restrict__ r) {
     const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= L.np) return;
+    if (p >= L.np || page_stopped(L, p)) return;
     const uint32_t f = L.flags[e];
     if (f & kActive) r[e] = cell_residual(L.nbr, p, c, f, u, b[e]);
 }
@@ -106,7 +109,7 @@ __device__ __forceinline__ int lut_page(const DevLevel& L, int x, int y, int z) 
 // coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only)
 __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const float* __restrict__ r_f, float* __restrict__ b_c) {
     const int p = xcd_page(C.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= C.np) return;
+    if (p >= C.np || page_stopped(C, p)) return;
     if (!(C.flags[e] & kActive)) return;
     const int x = C.base[p * 3 + 0] + (c >> 6), y = C.base[p * 3 + 1] + ((c >> 3) & 7), z = C.base[p * 3 + 2] + (c & 7);
     const int fx = 2 * x - 2, fy = 2 * y - 2, fz = 2 * z - 2;
@@ -124,7 +127,7 @@ __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const 
 // fine u += sum_{2x2x2} w u_coarse (Active fine cells only)
 __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, float* __restrict__ u_f, const float* __restrict__ u_c) {
     const int p = xcd_page(F.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= F.np) return;
+    if (p >= F.np || page_stopped(F, p)) return;
     if (!(F.flags[e] & kActive)) return;
     const int x = F.base[p * 3 + 0] + (c >> 6), y = F.base[p * 3 + 1] + ((c >> 3) & 7), z = F.base[p * 3 + 2] + (c & 7);
     const int cx = (x + 1) >> 1, cy = (y + 1) >> 1, cz = (z + 1) >> 1;
@@ -143,7 +146,7 @@ __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, float* 
 // PCG operator: out = L v on Active|Interface cells (double accumulation), 0 elsewhere (fused Project)
 __global__ void __launch_bounds__(256) k_laplace_pcg(DevLevel L, const float* __restrict__ v, float* __restrict__ out) {
     const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= L.np) return;
+    if (p >= L.np || page_stopped(L, p)) return;
     const uint32_t f = L.flags[e];
     float res = 0.f;
     if (f & (kActive | kInterface)) {
@@ -280,7 +283,7 @@ __global__ void __launch_bounds__(256) k_seg_finish(const double* __restrict__ d
 // host launchers
 static inline int grid_for(size_t n) { size_t g = (n + 255) / 256; return (int)(g > 4096 ? 4096 : (g == 0 ? 1 : g)); }
 
-void upload_constants() {
+static void upload_constants_once() {
     float om[8];
     const float omega = (float)(2. / 3.);
     om[0] = 0.f;
@@ -294,6 +297,12 @@ void upload_constants() {
     hdd_check(hipMemcpyToSymbol(HIP_SYMBOL(c_omega_fc), om, sizeof(om)));
     hdd_check(hipMemcpyToSymbol(HIP_SYMBOL(c_rweights), rw, sizeof(rw)));
     hdd_check(hipMemcpyToSymbol(HIP_SYMBOL(c_pweights), pw, sizeof(pw)));
+}
+
+// the stencil constants are the same for every context: uploaded once per process
+void upload_constants() {
+    static std::once_flag once;
+    std::call_once(once, upload_constants_once);
 }
 
 void launch_smooth(const DevLevel& L, const float* u_in, float* u_out, const float* b, bool interior, bool boundary, hipStream_t s) {

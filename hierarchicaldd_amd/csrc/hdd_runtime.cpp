@@ -11,6 +11,7 @@ DevLevel LevelBuffers::view() const {
     v.np = host.npages();
     v.nbx = host.nbx; v.nby = host.nby; v.nbz = host.nbz;
     v.flags = flags.p; v.nbr = nbr.p; v.base = base.p; v.lut = lut.p;
+    v.prob = prob.p; v.active = active;
     return v;
 }
 
@@ -155,10 +156,17 @@ void Solver::merge_levels() {
 void Solver::build_device() {
     const int NP = nprob();
     for (auto& q : prob) q->factorize();
-    hdd_check(hipStreamCreate(&stream), "hipStreamCreate");
+    // non-blocking: contexts of a server run side by side without the null stream serialising them
+    hdd_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     upload_constants();
     merge_levels();
     upload_levels();
+    for (int l = 0; l < (int)lv.size(); ++l) {
+        std::vector<int> pp((size_t)lv[l].host.npages() + 1, -1);
+        for (int p = 0; p < NP; ++p)
+            for (int g = pg0[l][p]; g < pg0[l][p + 1]; ++g) pp[g] = p;
+        lv[l].prob.upload(pp);
+    }
     const PagedLevel& L0 = lv[0].host;
     const int np0 = L0.npages();
     {
@@ -178,7 +186,7 @@ void Solver::build_device() {
             for (int g = pg0[0][p]; g < pg0[0][p + 1]; ++g) pp[g] = p;
         }
         seg_pg0.upload(pg0[0]); seg_parts.upload(parts); page_prob.upload(pp);
-        active.alloc(NP);
+        active.upload(std::vector<int>(NP, 1));
         dpart.alloc((size_t)NP * max_parts); fpart.alloc((size_t)NP * max_parts);
         scal.alloc((size_t)NP * 8); dscal.alloc((size_t)NP * 4);
     }
@@ -198,11 +206,12 @@ void Solver::build_device() {
     std::vector<const SigmaHost*> hosts;
     std::vector<const SigmaFactors*> facs;
     for (auto& q : prob) { hosts.push_back(&q->sigma); facs.push_back(q->factors.get()); }
-    sigma = make_sigma_device(hosts, facs, P.sig_s, P.sig_mg);
-    hdd_check(hipDeviceSynchronize(), "setup sync");
+    sigma = make_sigma_device(hosts, facs, P.sig_s, P.sig_mg, stream);
+    hdd_check(hipStreamSynchronize(stream), "setup sync");
 }
 
 Solver::~Solver() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     sigma.reset();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -282,7 +291,49 @@ void Solver::vcycle() {
 
 // Apply_Preconditioner (DDS/SPGrid_Domain_Decomposition_Preconditioner.h:88-128),
 // z = channel 0, rhs = channel 1 (PCG residual r), tmp k = channel 2.
+// The apply is a fixed sequence of launches for a given state of the
+// smoothing ping-pong buffers and of the skip mode, so with use_graph it is
+// captured once into a HIP graph per such state and replayed: one graph launch
+// instead of ~2000 kernel launches per PCG iteration. Nothing in it allocates,
+// copies or clears memory through the runtime (clears are fill kernels).
+// Off by default: the replay is bit-identical but measured no faster (the
+// iteration is bound by the GPU's dispatch of dependent kernels, not by host
+// launches), and a capture fails if another thread of the process uses the
+// null stream meanwhile (the server's setup workers do).
 void Solver::apply_preconditioner() {
+    if (!use_graph) { record_apply(); return; }
+    std::vector<int> key;
+    for (auto& L : lv) key.push_back(L.cur);
+    key.push_back(lv[0].active != nullptr);
+    for (auto& g : graphs)
+        if (g.key == key) {
+            hdd_check(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
+            for (size_t l = 0; l < lv.size(); ++l) lv[l].cur = g.cur_out[l];
+            return;
+        }
+    Graph g;
+    g.key = key;
+    hipGraph_t graph = nullptr;
+    hdd_check(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    try {
+        record_apply();
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    hdd_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hdd_check(e, "hipGraphInstantiate");
+    for (auto& L : lv) g.cur_out.push_back(L.cur);
+    for (size_t l = 0; l < lv.size(); ++l) lv[l].cur = key[l];  // capture recorded the launches without running them
+    graphs.push_back(g);
+    hdd_check(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
+    for (size_t l = 0; l < lv.size(); ++l) lv[l].cur = g.cur_out[l];
+}
+
+void Solver::record_apply() {
     LevelBuffers& L0 = lv[0];
     const size_t n0 = L0.cells();
     launch_fill(L0.u[L0.cur].p, 0.f, n0, stream);
@@ -308,7 +359,8 @@ void Solver::interface_scatter() {
 // for every problem of the batch, x starting at zero, with the stopping rule
 // max|r| <= tol[p]. Each problem has its own scalars; once it stops, its x
 // and r are frozen (the shared launches still visit its pages).
-void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out) {
+void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out,
+                       const std::function<void(int)>& on_stop) {
     const int NP = nprob();
     if ((int)tol.size() != NP || (int)max_it.size() != NP) throw std::runtime_error("pcg: one tolerance and iteration limit per problem");
     LevelBuffers& L0 = lv[0];
@@ -326,6 +378,22 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
     std::vector<int> act(NP, 1);
     hdd_check(hipMemcpyAsync(scal.p, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
+    // while the batch iterates, the stencil kernels and exact solves skip the problems that have stopped
+    struct SkipStopped {
+        Solver& S;
+        explicit SkipStopped(Solver& s) : S(s) { set(S.active.p); }
+        ~SkipStopped() {
+            set(nullptr);
+            std::vector<int> ones(S.nprob(), 1);
+            (void)hipMemcpyAsync(S.active.p, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice, S.stream);
+            (void)hipStreamSynchronize(S.stream);
+        }
+        void set(const int* a) {
+            for (auto& L : S.lv) L.active = a;
+            if (S.sigma) S.sigma->set_active(a);
+        }
+    };
+    std::unique_ptr<SkipStopped> skip(NP > 1 ? new SkipStopped(*this) : nullptr);
     launch_fill(x.p, 0.f, n0, stream);
     hdd_check(hipStreamSynchronize(stream));
     auto t0 = std::chrono::steady_clock::now();
@@ -352,6 +420,7 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
             act[p] = 0;
             --n_active;
             changed = true;
+            if (on_stop) on_stop(p);
         }
         if (n_active == 0) break;
         // act is not modified again before the next synchronisation
@@ -382,7 +451,8 @@ void Solver::download_x(int p, float* host) {
     if (p < 0 || p >= nprob()) throw std::runtime_error("no such problem in the batch");
     hdd_check(hipStreamSynchronize(stream), "sync");
     const size_t first = (size_t)pg0[0][p] * kPageCells, n = (size_t)(pg0[0][p + 1] - pg0[0][p]) * kPageCells;
-    hdd_check(hipMemcpy(host, x.p + first, n * sizeof(float), hipMemcpyDeviceToHost), "download x");
+    hdd_check(hipMemcpyAsync(host, x.p + first, n * sizeof(float), hipMemcpyDeviceToHost, stream), "download x");
+    hdd_check(hipStreamSynchronize(stream), "download x");
 }
 
 }  // namespace hdd

@@ -17,6 +17,7 @@
 //    problem's x and r are frozen while the others continue.
 // One problem is the special case of a batch of one.
 #pragma once
+#include <functional>
 #include <memory>
 #include <vector>
 #include "hdd_device.h"
@@ -39,12 +40,24 @@ struct DBuf {
     DBuf& operator=(const DBuf&) = delete;
     DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
     ~DBuf() { if (p) (void)hipFree(p); }
+    // The clear and the upload go through the null stream, which does not order
+    // the contexts' non-blocking streams: both wait for completion before returning.
     void alloc(size_t count) {
         if (p) { (void)hipFree(p); p = nullptr; }
         n = count;
-        if (count) { hdd_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc"); hdd_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset"); }
+        if (count) {
+            hdd_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+            hdd_check(hipMemset(p, 0, count * sizeof(T)), "hipMemset");
+            hdd_check(hipStreamSynchronize(nullptr), "hipMemset");
+        }
     }
-    void upload(const T* h, size_t count) { alloc(count); if (count) hdd_check(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice), "upload"); }
+    void upload(const T* h, size_t count) {
+        alloc(count);
+        if (count) {
+            hdd_check(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice), "upload");
+            hdd_check(hipStreamSynchronize(nullptr), "upload");
+        }
+    }
     void upload(const std::vector<T>& v) { upload(v.data(), v.size()); }
     std::vector<T> download() const {
         std::vector<T> v(n);
@@ -79,6 +92,8 @@ struct LevelBuffers {
     PagedLevel host;
     DBuf<uint32_t> flags;
     DBuf<int> nbr, base, lut;
+    DBuf<int> prob;                 // problem of each page (zero page: -1)
+    const int* active = nullptr;    // set while a batch PCG runs: its per-problem flags
     DBuf<float> u[2], b, r;
     int cur = 0;  // which u buffer holds the current iterate
     DevLevel view() const;
@@ -108,6 +123,7 @@ public:
     hipStream_t stream = nullptr;
     double setup_seconds = 0;
     bool host_only = false;
+    bool use_graph = false;            // replay the preconditioner apply from captured HIP graphs (opt-in)
 
     // one problem; host_only: run the host setup and skip every device allocation (introspection)
     Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b,
@@ -124,12 +140,22 @@ public:
     void smooth0(int mode);
     void apply_preconditioner();  // z <- M^{-1} r
     int pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds);
-    // every problem from x = 0; tol[p], max_it[p] per problem
-    void pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out);
+    // every problem from x = 0; tol[p], max_it[p] per problem. on_stop(p), when
+    // given, runs on this thread as soon as problem p has stopped (its x is final
+    // and the stream is idle), while the others continue.
+    void pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out,
+                   const std::function<void(int)>& on_stop = nullptr);
     // x of problem p (its level-0 pages in its own page order), n_pages(p)*256 floats
     void download_x(int p, float* host);
 
 private:
+    struct Graph {
+        std::vector<int> key;          // ping-pong state of every level, skip mode
+        std::vector<int> cur_out;      // ping-pong state after the apply
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+    void record_apply();
     void build_device();
     void merge_levels();
     void upload_levels();

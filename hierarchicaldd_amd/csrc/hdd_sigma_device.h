@@ -26,10 +26,13 @@ struct TriPlan {
     size_t factor_nnz = 0;
     // per-height ranges and launch parameters (host copies)
     std::vector<int> sn_begin, sl_begin, sl_rows, row_begin, ci_begin, bi_begin, max_sz, big, fused, G, Gc, Gf, Gb;
-    DBuf<int> orig_of, s0, sz, nr, roff, srows, sn_of, sn_list, sl_sn, sl_r0, rowlist, inc_ptr, ci_sn, ci_a0, ci_a1, bi_sn, bi_c, bi_j0;
+    DBuf<int> orig_of, s0, sz, nr, roff, srows, sn_of, sn_prob, sn_list, sl_sn, sl_r0, rowlist, inc_ptr, ci_sn, ci_a0, ci_a1, bi_sn, bi_c, bi_j0;
     DBuf<long long> loff, doff, cofs, cpos, bofs;
     DBuf<float> diag, diagT, l21, c, bpart, t, y;
-    void build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets);
+    // prob_of[m]: problem of factor m (empty: all problem 0)
+    void build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, const std::vector<int>& prob_of = {});
+    // per-problem flags (device, 1 = iterating): supernodes of stopped problems are skipped; nullptr: none skipped
+    const int* active = nullptr;
     void solve(const float* b_orig, float* x_orig, hipStream_t s);
     size_t bytes() const;
 };
@@ -101,12 +104,15 @@ struct SigmaDevice {
     void smooth(int l, hipStream_t s);
     void residual(int l, hipStream_t s);
     void vcycle(hipStream_t s, bool first);
+    // per-problem flags of the batch PCG (device): exact solves skip stopped problems; nullptr: skip nothing
+    void set_active(const int* active);
 };
 
 // Device interface solver of a batch: the problems' operators stacked
 // block-diagonally (problem order), from their host operators and factors.
+// The setup's own device work (the bottom self-check) runs on `stream`.
 std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost*>& hosts, const std::vector<const SigmaFactors*>& factors,
-                                               int smoothing, int n_mg);
+                                               int smoothing, int n_mg, hipStream_t stream);
 void sigma_vcycles(SigmaDevice& S, hipStream_t s);
 void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s);
 void launch_interface_scatter(SigmaDevice& S, const int* if_index, float* z, hipStream_t s);

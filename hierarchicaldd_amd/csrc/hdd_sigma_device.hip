@@ -42,15 +42,25 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+// In a batch, supernodes of problems that have stopped iterating are skipped
+// (active == nullptr: nothing is skipped). Every skip is uniform over the
+// workgroup, wavefront or lane group that owns the supernode or row.
+__device__ __forceinline__ bool sn_stopped(const int* __restrict__ sn_prob, const int* __restrict__ active, int g) {
+    return active && !active[sn_prob[g]];
+}
+
 // forward reduce (heights with big supernodes): t_i = b_i - sum of the
 // contributions at row i's incidences, G lanes per row
 template <int G>
 __global__ void __launch_bounds__(256) k_tri_fwd_reduce(const int* __restrict__ rows, int nrows, const int* __restrict__ inc_ptr,
                                                         const long long* __restrict__ cpos, const float* __restrict__ c,
-                                                        const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ t) {
+                                                        const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ t,
+                                                        const int* __restrict__ sn_of, const int* __restrict__ sn_prob,
+                                                        const int* __restrict__ active) {
     const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w];
+    if (sn_stopped(sn_prob, active, sn_of[i])) return;
     float s = 0.f;
     for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += G) s += c[cpos[p]];
     s = group_sum<G>(s);
@@ -67,9 +77,11 @@ __global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, cons
                                                     const long long* __restrict__ doff, const float* __restrict__ diag,
                                                     const float* __restrict__ tg, const int* __restrict__ inc_ptr,
                                                     const long long* __restrict__ cpos, const float* __restrict__ c,
-                                                    const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ y) {
+                                                    const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ y,
+                                                    const int* __restrict__ sn_prob, const int* __restrict__ active) {
     extern __shared__ float ts[];
     const int g = sl_sn[q0 + blockIdx.x], r0 = sl_r0[q0 + blockIdx.x], n = sz[g], base = s0[g];
+    if (sn_stopped(sn_prob, active, g)) return;
     const int rend = min(n, r0 + slice_rows);
     for (int k = threadIdx.x; k < rend; k += 256) {
         const int i = base + k;
@@ -98,11 +110,13 @@ template <int G>
 __global__ void __launch_bounds__(256) k_tri_fwd_contrib(int i0, int i1, const int* __restrict__ ci_sn, const int* __restrict__ ci_a0,
                                                          const int* __restrict__ ci_a1, const int* __restrict__ s0, const int* __restrict__ sz,
                                                          const long long* __restrict__ loff, const long long* __restrict__ cofs,
-                                                         const float* __restrict__ l21, const float* __restrict__ y, float* __restrict__ c) {
+                                                         const float* __restrict__ l21, const float* __restrict__ y, float* __restrict__ c,
+                                                         const int* __restrict__ sn_prob, const int* __restrict__ active) {
     // a workgroup per item; its rows go to the 256/G lane groups in turn
     const int it = i0 + blockIdx.x;
     if (it >= i1) return;
     const int g = ci_sn[it], n = sz[g], a1 = ci_a1[it];
+    if (sn_stopped(sn_prob, active, g)) return;
     const float* L = l21 + loff[g];
     const float* yy = y + s0[g];
     float* cg = c + cofs[g];
@@ -123,10 +137,12 @@ __global__ void __launch_bounds__(256) k_tri_bwd_tiles(int i0, int i1, const int
                                                        const int* __restrict__ bi_j0, const int* __restrict__ sz, const int* __restrict__ nr,
                                                        const long long* __restrict__ loff, const int* __restrict__ roff,
                                                        const int* __restrict__ srows, const long long* __restrict__ bofs,
-                                                       const float* __restrict__ l21, const float* __restrict__ z, float* __restrict__ bpart) {
+                                                       const float* __restrict__ l21, const float* __restrict__ z, float* __restrict__ bpart,
+                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
     const int it = i0 + blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (it >= i1) return;
     const int g = bi_sn[it], n = sz[g], cc = bi_c[it], j = bi_j0[it] + lane;
+    if (sn_stopped(sn_prob, active, g)) return;
     const int a0 = cc * TriPlanHost::kTile, m = min(nr[g] - a0, TriPlanHost::kTile);
     // lane q holds z at the tile's row q; the row loop then issues only independent L21 loads
     const float zq = lane < m ? z[srows[roff[g] + a0 + lane]] : 0.f;
@@ -143,10 +159,12 @@ template <int G>
 __global__ void __launch_bounds__(256) k_tri_bwd_reduce(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
                                                         const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
                                                         const long long* __restrict__ bofs, const float* __restrict__ bpart,
-                                                        const float* __restrict__ y, float* __restrict__ t) {
+                                                        const float* __restrict__ y, float* __restrict__ t,
+                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
     if (w >= nrows) return;
     const int i = rows[w], g = sn_of[i], n = sz[g], k = i - s0[g];
+    if (sn_stopped(sn_prob, active, g)) return;
     const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
     const float* P = bpart + bofs[g] + k;
     float s = 0.f;
@@ -166,9 +184,11 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
                                                     const long long* __restrict__ doff, const float* __restrict__ diagT,
                                                     const float* __restrict__ tg, const long long* __restrict__ bofs,
                                                     const float* __restrict__ bpart, const int* __restrict__ orig_of,
-                                                    const float* __restrict__ y, float* __restrict__ z, float* __restrict__ x) {
+                                                    const float* __restrict__ y, float* __restrict__ z, float* __restrict__ x,
+                                                    const int* __restrict__ sn_prob, const int* __restrict__ active) {
     extern __shared__ float ts[];
     const int g = sl_sn[q0 + blockIdx.x], r0 = sl_r0[q0 + blockIdx.x], n = sz[g], base = s0[g];
+    if (sn_stopped(sn_prob, active, g)) return;
     const int rend = min(n, r0 + slice_rows);
     const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
     for (int k = r0 + (int)threadIdx.x; k < n; k += 256) {
@@ -203,9 +223,11 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int q0, const int* __rest
                                                        const int* __restrict__ inc_ptr, const long long* __restrict__ cpos,
                                                        const int* __restrict__ orig_of, const float* __restrict__ b,
                                                        const long long* __restrict__ loff, const long long* __restrict__ cofs,
-                                                       const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y) {
+                                                       const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
+                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
     __shared__ float ts[TriPlanHost::kSliceRows], ys[TriPlanHost::kSliceRows];
     const int g = sn_list[q0 + blockIdx.x], n = sz[g], base = s0[g], m = nr[g];
+    if (sn_stopped(sn_prob, active, g)) return;
     for (int k = threadIdx.x; k < n; k += 256) {
         const int i = base + k;
         float sum = 0.f;
@@ -252,12 +274,14 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int q0, const int* __rest
                                                        const long long* __restrict__ loff, const float* __restrict__ l21,
                                                        const long long* __restrict__ doff, const float* __restrict__ diagT,
                                                        const int* __restrict__ orig_of, const float* __restrict__ y,
-                                                       float* __restrict__ z, float* __restrict__ x) {
+                                                       float* __restrict__ z, float* __restrict__ x,
+                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
     constexpr int kChunk = 1024;
     __shared__ float zr[kChunk];
     __shared__ float part[4][TriPlanHost::kSliceRows];
     __shared__ float ts[TriPlanHost::kSliceRows];
     const int g = sn_list[q0 + blockIdx.x], n = sz[g], base = s0[g], m = nr[g];
+    if (sn_stopped(sn_prob, active, g)) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int* R = srows + roff[g];
     const float* L = l21 + loff[g];
@@ -424,11 +448,20 @@ static Csr transpose(const Csr& A) {
     return T;
 }
 
-void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets) {
+void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, const std::vector<int>& prob_of) {
     TriPlanHost P;
     build_triplan(F, offsets, P);
     n = P.n; H = P.H; factor_nnz = P.factor_nnz;
     if (n == 0) return;
+    {
+        // problem of each supernode (supernodes are numbered factor by factor, in order)
+        std::vector<int> sp(P.s0.size(), 0);
+        size_t g = 0;
+        for (size_t m = 0; m < F.size(); ++m)
+            for (size_t k = 0; k < F[m]->sn.size(); ++k, ++g) sp[g] = prob_of.empty() ? 0 : prob_of[m];
+        if (g != sp.size()) throw std::runtime_error("triangular-solve plan: supernode count mismatch");
+        sn_prob.upload(sp);
+    }
     sn_begin = P.sn_begin; sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
     max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
     orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
@@ -441,7 +474,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
 }
 
 size_t TriPlan::bytes() const {
-    return (orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
+    return (orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_prob.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
             ci_a1.n + bi_sn.n + bi_c.n + bi_j0.n) * 4 +
            (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4;
 }
@@ -468,31 +501,32 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     if (n == 0) return;
     float* yf = y.p;
     float* zb = y.p + n;
+    const int* SP = sn_prob.p;
     for (int h = 0; h <= H; ++h) {
         if (fused[h]) {
             const int q0 = sn_begin[h], nq = sn_begin[h + 1] - sn_begin[h];
             if (nq)
                 HDD_SN_DISPATCH(G[h], k_tri_fwd_small, nq, 0, s, q0, (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p,
                                 (const long long*)doff.p, (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p,
-                                (const int*)orig_of.p, b, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf);
+                                (const int*)orig_of.p, b, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
             continue;
         }
         const int nrows = row_begin[h + 1] - row_begin[h];
         const int* rows = rowlist.p + row_begin[h];
         if (big[h] && nrows)
             HDD_ROWS_DISPATCH(Gf[h], k_tri_fwd_reduce, nrows, s, rows, nrows, (const int*)inc_ptr.p, (const long long*)cpos.p,
-                              (const float*)c.p, (const int*)orig_of.p, b, t.p);
+                              (const float*)c.p, (const int*)orig_of.p, b, t.p, (const int*)sn_of.p, SP, active);
         const int q0 = sl_begin[h], nq = sl_begin[h + 1] - sl_begin[h];
         if (nq)
             HDD_SN_DISPATCH(G[h], k_tri_fwd_sn, nq, (size_t)max_sz[h] * sizeof(float), s, q0, sl_rows[h], (const int*)sl_sn.p, (const int*)sl_r0.p,
                             (const int*)s0.p, (const int*)sz.p, (const long long*)doff.p, (const float*)diag.p,
                             big[h] ? (const float*)t.p : (const float*)nullptr, (const int*)inc_ptr.p, (const long long*)cpos.p,
-                            (const float*)c.p, (const int*)orig_of.p, b, yf);
+                            (const float*)c.p, (const int*)orig_of.p, b, yf, SP, active);
         const int i0 = ci_begin[h], ni = ci_begin[h + 1] - ci_begin[h];
         if (ni)
             HDD_SN_DISPATCH(Gc[h], k_tri_fwd_contrib, ni, 0, s, i0, i0 + ni, (const int*)ci_sn.p, (const int*)ci_a0.p, (const int*)ci_a1.p,
                             (const int*)s0.p, (const int*)sz.p, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p,
-                            (const float*)yf, c.p);
+                            (const float*)yf, c.p, SP, active);
     }
     for (int h = H; h >= 0; --h) {
         if (fused[h]) {
@@ -500,25 +534,25 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
             if (nq)
                 HDD_SN_DISPATCH(G[h], k_tri_bwd_small, nq, 0, s, q0, (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p,
                                 (const int*)roff.p, (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
-                                (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x);
+                                (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
             continue;
         }
         const int i0 = bi_begin[h], ni = bi_begin[h + 1] - bi_begin[h];
         if (ni)
             hipLaunchKernelGGL(k_tri_bwd_tiles, dim3(blocks_for(ni, 4)), dim3(256), 0, s, i0, i0 + ni, (const int*)bi_sn.p, (const int*)bi_c.p,
                                (const int*)bi_j0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)loff.p, (const int*)roff.p,
-                               (const int*)srows.p, (const long long*)bofs.p, (const float*)l21.p, (const float*)zb, bpart.p);
+                               (const int*)srows.p, (const long long*)bofs.p, (const float*)l21.p, (const float*)zb, bpart.p, SP, active);
         const int nrows = row_begin[h + 1] - row_begin[h];
         const int* rows = rowlist.p + row_begin[h];
         if (big[h] && nrows)
             HDD_ROWS_DISPATCH(Gb[h], k_tri_bwd_reduce, nrows, s, rows, nrows, (const int*)sn_of.p, (const int*)s0.p, (const int*)sz.p,
-                              (const int*)nr.p, (const long long*)bofs.p, (const float*)bpart.p, (const float*)yf, t.p);
+                              (const int*)nr.p, (const long long*)bofs.p, (const float*)bpart.p, (const float*)yf, t.p, SP, active);
         const int q0 = sl_begin[h], nq = sl_begin[h + 1] - sl_begin[h];
         if (nq)
             HDD_SN_DISPATCH(G[h], k_tri_bwd_sn, nq, (size_t)max_sz[h] * sizeof(float), s, q0, sl_rows[h], (const int*)sl_sn.p, (const int*)sl_r0.p,
                             (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diagT.p,
                             big[h] ? (const float*)t.p : (const float*)nullptr, (const long long*)bofs.p, (const float*)bpart.p,
-                            (const int*)orig_of.p, (const float*)yf, zb, x);
+                            (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
     }
 }
 
@@ -607,24 +641,24 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
 
 // Compares the bottom solve with the reference's 200 sweeps on one random
 // right-hand side (level-0 buffers of the bottom level are scratch here).
-static double bottom_self_check(SigmaDevice& S) {
+static double bottom_self_check(SigmaDevice& S, hipStream_t s) {
     SigmaLevel& B = S.lv[S.levels - 1];
     const int n = B.n_if;
     std::vector<float> b(n);
     uint64_t st = 0x243f6a8885a308d3ULL;
     for (int i = 0; i < n; ++i) { st = st * 6364136223846793005ULL + 1442695040888963407ULL; b[i] = (float)((double)(st >> 40) / 16777216.0 - 0.5); }
-    hdd_check(hipMemcpy(B.b.p, b.data(), n * sizeof(float), hipMemcpyHostToDevice));
-    S.bottom(0, true);
+    hdd_check(hipMemcpyAsync(B.b.p, b.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    S.bottom(s, true);
     std::vector<float> ud(n), ul(n);
-    hdd_check(hipMemcpy(ud.data(), B.u.p, n * sizeof(float), hipMemcpyDeviceToHost));
-    launch_fill(B.u.p, 0.f, n, 0);
-    for (int i = 0; i < 200; ++i) S.smooth(S.levels - 1, 0);
-    hdd_check(hipMemcpy(ul.data(), B.u.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    hdd_check(hipMemcpyAsync(ud.data(), B.u.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    launch_fill(B.u.p, 0.f, n, s);
+    for (int i = 0; i < 200; ++i) S.smooth(S.levels - 1, s);
+    hdd_check(hipMemcpyAsync(ul.data(), B.u.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    launch_fill(B.u.p, 0.f, n, s);
+    launch_fill(B.b.p, 0.f, n, s);
+    hdd_check(hipStreamSynchronize(s));
     double dmax = 0, umax = 0;
     for (int i = 0; i < n; ++i) { dmax = std::max(dmax, (double)std::fabs(ud[i] - ul[i])); umax = std::max(umax, (double)std::fabs(ul[i])); }
-    launch_fill(B.u.p, 0.f, n, 0);
-    launch_fill(B.b.p, 0.f, n, 0);
-    hdd_check(hipDeviceSynchronize());
     return umax > 0 ? dmax / umax : dmax;
 }
 
@@ -638,7 +672,7 @@ static void append_rows(Csr& M, const Csr& A, int col_off) {
 }
 
 std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost*>& hosts, const std::vector<const SigmaFactors*>& factors,
-                                               int smoothing, int n_mg) {
+                                               int smoothing, int n_mg, hipStream_t stream) {
     const int NP = (int)hosts.size();
     if (NP == 0 || (int)factors.size() != NP) throw std::runtime_error("make_sigma_device: one host operator set and factor set per problem");
     std::unique_ptr<SigmaDevice> S(new SigmaDevice());
@@ -666,7 +700,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         Csr air, arr, pr;
         air.cols = L.n_if; arr.cols = L.n_if;
         std::vector<const SupernodalFactor*> fi, fr;
-        std::vector<int> oi, orr;
+        std::vector<int> oi, orr, pi, prr;
         int n_int = 0;
         for (int p = 0; p < NP; ++p) {
             const SigmaHost& H = *hosts[p];
@@ -677,11 +711,12 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
                     const int rp = F.rep[s];
                     fi.push_back(canonical(H.Aii[l][rp], &F.Aii[rp]));
                     oi.push_back(n_int);
+                    pi.push_back(p);
                 }
                 n_int += H.Aii[l][s].rows;
             }
             append_rows(arr, H.Arr[l], ifoff[l][p]);
-            if (H.n_if[l] > 0) { fr.push_back(&F.Arr); orr.push_back(ifoff[l][p]); }
+            if (H.n_if[l] > 0) { fr.push_back(&F.Arr); orr.push_back(ifoff[l][p]); prr.push_back(p); }
             if (l + 1 < NL) append_rows(pr, H.Prr[l], ifoff[l + 1][p]);
         }
         L.n_int = n_int;
@@ -692,8 +727,8 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         L.AirT.upload(transpose(air));
         L.Arr.upload(arr);
         if (l + 1 < NL) { pr.cols = ifoff[l + 1][NP]; L.P.upload(pr); L.PT.upload(transpose(pr)); }
-        L.Lii.build(fi, oi);
-        L.Lrr.build(fr, orr);
+        L.Lii.build(fi, oi, pi);
+        L.Lrr.build(fr, orr, prr);
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
         S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
@@ -713,7 +748,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
     if (NL > 0 && any_if && all_direct) {
         const int n = ifoff[lb][NP];
         std::vector<const SupernodalFactor*> fb;
-        std::vector<int> ob, pos(n), rprob(n), m0(NP + 1, 0), prow0(NP), pn(NP), mrow0, mlen;
+        std::vector<int> ob, pb, pos(n), rprob(n), m0(NP + 1, 0), prow0(NP), pn(NP), mrow0, mlen;
         std::vector<long long> pvofs(NP, 0), mvofs;
         std::vector<float> V, SV;
         std::vector<double> w1, w2;
@@ -726,7 +761,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
             const int k = np > 0 ? (int)F.spec.lambda.size() : 0;
             m0[p + 1] = m0[p] + k;
             if (np == 0) continue;
-            fb.push_back(&F.bottom); ob.push_back(boff);
+            fb.push_back(&F.bottom); ob.push_back(boff); pb.push_back(p);
             for (int i = 0; i < np; ++i) { pos[r0 + i] = boff + F.bottom_n_int + i; rprob[r0 + i] = p; }
             for (int j = 0; j < k; ++j) {
                 mrow0.push_back(r0); mlen.push_back(np); mvofs.push_back((long long)V.size() + (long long)j * np);
@@ -738,7 +773,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
             S->spec_nch = std::max(S->spec_nch, (np + kSpecChunk - 1) / kSpecChunk);
             boff += F.bottom_n_int + np;
         }
-        S->Lbottom.build(fb, ob);
+        S->Lbottom.build(fb, ob, pb);
         S->bot_rhs.alloc(boff); S->bot_sol.alloc(boff);
         S->bot_pos.upload(pos);
         S->bottom_k = m0[NP];
@@ -755,7 +790,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         S->plan_bytes += S->Lbottom.bytes() + S->spec_V.n * 4 + S->spec_SV.n * 4;
     }
     if (S->bottom_direct) {
-        S->bottom_check = bottom_self_check(*S);
+        S->bottom_check = bottom_self_check(*S, stream);
         // the reference's sweeps stay in charge if the shortcut disagrees with them
         if (bottom_mode() != "direct" && !(S->bottom_check < 1e-3)) S->bottom_direct = false;
     }
@@ -833,6 +868,11 @@ void SigmaDevice::vcycle(hipStream_t s, bool first) {
                                (const float*)lv[l + 1].u.p, F.u.p);
         for (int i = 0; i < smoothing; ++i) smooth(l, s);
     }
+}
+
+void SigmaDevice::set_active(const int* a) {
+    for (auto& L : lv) { L.Lii.active = a; L.Lrr.active = a; }
+    Lbottom.active = a;
 }
 
 void sigma_vcycles(SigmaDevice& S, hipStream_t s) {

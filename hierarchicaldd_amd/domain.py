@@ -129,10 +129,13 @@ def _padded(n):
     return (-(-n // BX) * BX, -(-n // BY) * BY, -(-n // BZ) * BZ)
 
 
-def sphere_obstacle_box(n: int, spheres, delta=None, rhs: np.ndarray | None = None, meta=None) -> SPGridDomain:
-    """Box of n^3 cells with a Dirichlet ring and Neumann sphere obstacles.
+def box_flags(n: int, spheres, drop_enclosed: bool = False) -> np.ndarray:
+    """Dense flags (padded to page multiples) of an n^3 box with a Dirichlet ring
+    and Neumann sphere obstacles (``spheres``: list of (cx, cy, cz, r)).
 
-    ``spheres`` is a list of (cx, cy, cz, r); ``delta`` is ((x,y,z), value)."""
+    ``drop_enclosed``: Active pockets sealed off from every Dirichlet cell by
+    obstacles (pure-Neumann, singular subproblems) are turned into obstacle
+    cells, so any right-hand side is solvable."""
     X, Y, Z = _padded(n)
     i = np.arange(X)[:, None, None]
     j = np.arange(Y)[None, :, None]
@@ -141,17 +144,36 @@ def sphere_obstacle_box(n: int, spheres, delta=None, rhs: np.ndarray | None = No
     interior = (i >= 1) & (i <= n - 2) & (j >= 1) & (j <= n - 2) & (k >= 1) & (k <= n - 2)
     obstacle = np.zeros((X, Y, Z), dtype=bool)
     for (cx, cy, cz, r) in spheres:
-        obstacle |= ((i - cx) ** 2 + (j - cy) ** 2 + (k - cz) ** 2) <= r * r
+        _mark_sphere(obstacle, cx, cy, cz, r)
     flags = np.zeros((X, Y, Z), dtype=np.uint32)
     flags[inside & ~interior] = DIRICHLET
-    flags[interior & ~obstacle] = ACTIVE
+    active = interior & ~obstacle
+    if drop_enclosed:
+        active = _connected_to_ring(active, n)
+    flags[active] = ACTIVE
     # Dirichlet cells with no Active face neighbour are dropped
     act = np.pad(flags == ACTIVE, 1)
     has_act = (act[0:X, 1:Y + 1, 1:Z + 1] | act[2:X + 2, 1:Y + 1, 1:Z + 1] | act[1:X + 1, 0:Y, 1:Z + 1] |
                act[1:X + 1, 2:Y + 2, 1:Z + 1] | act[1:X + 1, 1:Y + 1, 0:Z] | act[1:X + 1, 1:Y + 1, 2:Z + 2])
     flags[(flags == DIRICHLET) & ~has_act] = 0
-    flags = _face_bits(flags)
-    b = np.zeros((X, Y, Z), dtype=np.float32)
+    return _face_bits(flags)
+
+
+def _mark_sphere(mask: np.ndarray, cx, cy, cz, r):
+    """mask |= |(i,j,k) - c|^2 <= r^2, evaluated over the sphere's bounding box only."""
+    lo = [max(0, int(np.floor(c - r))) for c in (cx, cy, cz)]
+    hi = [min(m, int(np.ceil(c + r)) + 1) for c, m in zip((cx, cy, cz), mask.shape)]
+    if any(h <= l for l, h in zip(lo, hi)):
+        return
+    i = np.arange(lo[0], hi[0])[:, None, None]
+    j = np.arange(lo[1], hi[1])[None, :, None]
+    k = np.arange(lo[2], hi[2])[None, None, :]
+    mask[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] |= ((i - cx) ** 2 + (j - cy) ** 2 + (k - cz) ** 2) <= r * r
+
+
+def domain_from_flags(n: int, flags: np.ndarray, delta=None, rhs: np.ndarray | None = None, meta=None) -> SPGridDomain:
+    """Page a dense flag array with a delta ((x,y,z), value) and/or a dense right-hand side on Active cells."""
+    b = np.zeros(flags.shape, dtype=np.float32)
     if rhs is not None:
         b[:n, :n, :n] = np.where((flags[:n, :n, :n] & ACTIVE) != 0, rhs, 0)
     if delta is not None:
@@ -160,6 +182,37 @@ def sphere_obstacle_box(n: int, spheres, delta=None, rhs: np.ndarray | None = No
             raise ValueError(f"delta position {(dx, dy, dz)} is not an Active cell")
         b[dx, dy, dz] = val
     return page_dense_grid(flags, b, (n + 1, n + 1, n + 1), meta)
+
+
+def sphere_obstacle_box(n: int, spheres, delta=None, rhs: np.ndarray | None = None, meta=None,
+                        drop_enclosed: bool = False) -> SPGridDomain:
+    """Box of n^3 cells with a Dirichlet ring and Neumann sphere obstacles.
+
+    ``spheres`` is a list of (cx, cy, cz, r); ``delta`` is ((x,y,z), value)."""
+    return domain_from_flags(n, box_flags(n, spheres, drop_enclosed), delta, rhs, meta)
+
+
+def _connected_to_ring(active: np.ndarray, n: int) -> np.ndarray:
+    """Active cells whose 6-connected component touches a face-neighbour of the Dirichlet ring."""
+    from scipy import ndimage
+    lab, _ = ndimage.label(active)
+    edge = np.zeros(active.shape, dtype=bool)
+    edge[1, :, :] = edge[n - 2, :, :] = True
+    edge[:, 1, :] = edge[:, n - 2, :] = True
+    edge[:, :, 1] = edge[:, :, n - 2] = True
+    keep = np.unique(lab[edge & active])
+    keep = keep[keep > 0]
+    return np.isin(lab, keep)
+
+
+def random_spheres(rng, n: int, count: int, rmin: float, rmax: float):
+    """count spheres with radius uniform in [rmin, rmax) * n and centres uniform in the box."""
+    out = []
+    for _ in range(count):
+        r = float(rng.uniform(rmin, rmax)) * n
+        c = rng.uniform(0, n, size=3)
+        out.append((float(c[0]), float(c[1]), float(c[2]), r))
+    return out
 
 
 @dataclass
@@ -197,3 +250,39 @@ def small_box(n: int = 32, sub: int = 16, vlevels: int = 3, seed: int = 0, sigma
     rhs = rng.standard_normal((n, n, n)).astype(np.float32)
     d = sphere_obstacle_box(n, [(c + 1, c - 2, c, max(2, n // 6))], rhs=rhs, meta={"workload": f"box{n}_sub{sub}"})
     return d, DDConfig(sub_size=sub, vlevels=vlevels, sigma_levels=sigma_levels)
+
+
+def batch_problems(count: int = 64, seed: int = 0, sub: int = 32, vlevels: int = 4):
+    """Uneven independent problems: edge n = 32 * round(exp(U[ln 64, ln 256]) / 32),
+    1-4 sphere obstacles of radius 0.1-0.3 n, a delta of 1.0 at a random Active
+    cell, sub_size 32 (enclosed pockets are dropped)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(32 * round(float(np.exp(rng.uniform(np.log(64), np.log(256)))) / 32))
+        spheres = random_spheres(rng, n, int(rng.integers(1, 5)), 0.1, 0.3)
+        flags = box_flags(n, spheres, drop_enclosed=True)
+        cells = np.flatnonzero((flags & ACTIVE) != 0)
+        dx, dy, dz = (int(v) for v in np.unravel_index(cells[int(rng.integers(0, len(cells)))], flags.shape))
+        d = domain_from_flags(n, flags, delta=((dx, dy, dz), 1.0), meta={"workload": f"batch{i}_n{n}", "n": n})
+        out.append((d, DDConfig(sub_size=sub, vlevels=vlevels)))
+    return out
+
+
+def served_request(rng, nmin: int = 32, nmax: int = 384, sub: int = 32, vlevels: int = 4):
+    """One request of the served stream: edge n log-uniform in [nmin, nmax] (rounded
+    to a multiple of 8), seeded spheres of radius 0.1-0.3 n added until a target
+    fraction U[0, 0.4) of the cells is blocked, an N(0,1) right-hand side on the
+    Active cells (enclosed pockets dropped)."""
+    n = int(8 * round(float(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))) / 8))
+    target = float(rng.uniform(0.0, 0.4))
+    spheres = []
+    blocked = np.zeros((n, n, n), dtype=bool)
+    while blocked.mean() < target and len(spheres) < 64:
+        sp = random_spheres(rng, n, 1, 0.1, 0.3)[0]
+        spheres.append(sp)
+        _mark_sphere(blocked, *sp)
+    rhs = rng.standard_normal((n, n, n)).astype(np.float32)
+    flags = box_flags(n, spheres, drop_enclosed=True)
+    return (domain_from_flags(n, flags, rhs=rhs, meta={"workload": f"request_n{n}", "n": n, "blocked_target": target}),
+            DDConfig(sub_size=sub, vlevels=vlevels))

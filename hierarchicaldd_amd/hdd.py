@@ -38,13 +38,23 @@ class PcgReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("final_norm", C.c_double)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("setup_threads", C.c_int), ("groups", C.c_int), ("max_batch", C.c_int), ("hbm_budget_bytes", C.c_double)]
+
+
+class RequestReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("final_norm", C.c_double), ("latency_s", C.c_double), ("setup_s", C.c_double),
+                ("queue_s", C.c_double), ("solve_s", C.c_double), ("batch_size", C.c_int)]
+
+
 EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
            "hdd_get_channel", "hdd_level_pages", "hdd_level_flags", "hdd_smooth", "hdd_vcycle",
            "hdd_apply_preconditioner", "hdd_sigma_info", "hdd_interface_dofs", "hdd_interface_coords",
            "hdd_interface_matrix", "hdd_sigma_vcycle", "hdd_sigma_smooth", "hdd_pcg_solve", "hdd_setup_seconds", "hdd_profile_sweep", "hdd_stats",
            "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats",
            "hdd_profile_solve", "hdd_create_batch", "hdd_batch_count", "hdd_batch_num_pages", "hdd_batch_page_offsets",
-           "hdd_batch_pcg_solve", "hdd_batch_get_x"]
+           "hdd_batch_pcg_solve", "hdd_batch_get_x", "hdd_create_batch_from", "hdd_server_create", "hdd_server_submit",
+           "hdd_server_wait", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph"]
 
 
 def lib():
@@ -89,8 +99,17 @@ def lib():
         L.hdd_batch_page_offsets.argtypes = [P, I, P]
         L.hdd_batch_pcg_solve.argtypes = [P, C.POINTER(PcgParams), C.POINTER(PcgReport), P, I]
         L.hdd_batch_get_x.argtypes = [P, I, P]
+        L.hdd_create_batch_from.argtypes = [C.POINTER(C.c_void_p), I, C.POINTER(C.c_void_p)]
+        L.hdd_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(C.c_void_p)]
+        L.hdd_server_submit.argtypes = [P, C.POINTER(ProblemDesc), C.POINTER(PcgParams), C.POINTER(C.c_int64)]
+        L.hdd_server_wait.argtypes = [P, C.c_int64, P, C.POINTER(RequestReport)]
+        L.hdd_server_destroy.argtypes = [P]
+        L.hdd_server_last_error.argtypes = [P]
+        L.hdd_server_last_error.restype = C.c_char_p
+        L.hdd_server_budget.argtypes = [P, C.POINTER(C.c_double)]
+        L.hdd_set_graph.argtypes = [P, I]
         for name in EXPORTS:
-            if name != "hdd_last_error":
+            if name not in ("hdd_last_error", "hdd_server_last_error"):
                 getattr(L, name).restype = I
         _lib = L
     return _lib
@@ -223,6 +242,10 @@ class DDSolver:
         self._check(lib().hdd_sigma_smooth(self.h, level, _ptr(b), _ptr(u)))
         return u
 
+    def set_graph(self, enable: bool):
+        """Replay the preconditioner apply from captured HIP graphs, or launch every kernel (default)."""
+        self._check(lib().hdd_set_graph(self.h, int(bool(enable))))
+
     def solve(self, tolerance: float, max_iterations: int = 5000):
         """PCG from x=0; returns (iterations, residual history, seconds)."""
         p = PcgParams(tolerance, max_iterations)
@@ -277,18 +300,25 @@ class DDSolver:
 
 
 class DDBatch(DDSolver):
-    """Several independent problems in one context (hdd_create_batch): one launch per stage for all of them."""
+    """Several independent problems in one context (hdd_create_batch): one launch per stage for all of them.
 
-    def __init__(self, problems):
+    ``problems`` is a list of (domain, config) pairs, or ``from_solvers`` a list of
+    DDSolver contexts whose host setup is reused (hdd_create_batch_from)."""
+
+    def __init__(self, problems=None, from_solvers=None):
         L = lib()
-        descs = (ProblemDesc * len(problems))()
-        self._keep = []
-        for i, (dom, cfg) in enumerate(problems):
-            d, keep = _desc(dom, cfg)
-            descs[i] = d
-            self._keep += keep
         h = C.c_void_p()
-        st = L.hdd_create_batch(descs, len(problems), C.byref(h))
+        self._keep = []
+        if from_solvers is not None:
+            arr = (C.c_void_p * len(from_solvers))(*[g.h.value for g in from_solvers])
+            st = L.hdd_create_batch_from(arr, len(from_solvers), C.byref(h))
+        else:
+            descs = (ProblemDesc * len(problems))()
+            for i, (dom, cfg) in enumerate(problems):
+                d, keep = _desc(dom, cfg)
+                descs[i] = d
+                self._keep += keep
+            st = L.hdd_create_batch(descs, len(problems), C.byref(h))
         if st != 0:
             raise HddError(f"hdd_create_batch failed ({st}): {L.hdd_last_error(None).decode()}")
         self.h = h
@@ -323,3 +353,51 @@ class DDBatch(DDSolver):
         out = np.zeros((self.problem_pages(p), 256), dtype=np.float32)
         self._check(lib().hdd_batch_get_x(self.h, p, _ptr(out)))
         return out
+
+
+class Server:
+    """Serving loop (hdd_server_*): submit problems as they arrive, wait for each result."""
+
+    def __init__(self, setup_threads: int = 2, groups: int = 2, max_batch: int = 16, hbm_budget: float = 0.0):
+        L = lib()
+        prm = ServerParams(setup_threads, groups, max_batch, float(hbm_budget))
+        h = C.c_void_p()
+        st = L.hdd_server_create(C.byref(prm), C.byref(h))
+        if st != 0:
+            raise HddError(f"hdd_server_create failed ({st}): {L.hdd_server_last_error(None).decode()}")
+        self.h = h
+        self.n_pages = {}
+
+    @property
+    def budget(self):
+        b = C.c_double()
+        lib().hdd_server_budget(self.h, C.byref(b))
+        return b.value
+
+    def submit(self, dom, cfg, tolerance: float, max_iterations: int = 500) -> int:
+        d, keep = _desc(dom, cfg)
+        rid = C.c_int64()
+        prm = PcgParams(float(tolerance), int(max_iterations))
+        st = lib().hdd_server_submit(self.h, C.byref(d), C.byref(prm), C.byref(rid))
+        if st != 0:
+            raise HddError(f"hdd_server_submit failed ({st}): {lib().hdd_server_last_error(self.h).decode()}")
+        self.n_pages[rid.value] = dom.n_pages
+        return rid.value
+
+    def wait(self, rid: int):
+        """Blocks until request rid is done: (status, x [n_pages, 256] in the submitted page order, report dict)."""
+        x = np.zeros((self.n_pages.pop(rid), 256), dtype=np.float32)
+        rep = RequestReport()
+        st = lib().hdd_server_wait(self.h, rid, _ptr(x), C.byref(rep))
+        if st not in (0, 4):
+            raise HddError(f"request {rid} failed ({st}): {lib().hdd_server_last_error(self.h).decode()}")
+        info = {k: getattr(rep, k) for k, _ in RequestReport._fields_}
+        return st, x, info
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hdd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

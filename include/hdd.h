@@ -71,6 +71,12 @@ hdd_status hdd_create(const hdd_problem_desc* desc, hdd_ctx** out);
  * flags and interface operators on machines without a GPU. */
 hdd_status hdd_create_host_only(const hdd_problem_desc* desc, hdd_ctx** out);
 
+/* Batch of the problems already set up in other contexts (hdd_create or
+ * hdd_create_host_only), in order: their host setup and factors are shared,
+ * not repeated, and the source contexts stay valid. Otherwise as
+ * hdd_create_batch (no reference counterpart). */
+hdd_status hdd_create_batch_from(hdd_ctx* const* ctxs, int count, hdd_ctx** out);
+
 /* Batch of independent problems in one context (no reference counterpart:
  * the reference solves one problem per process). Each problem is set up as
  * by hdd_create; their pages share one device grid and their interface
@@ -103,6 +109,13 @@ hdd_status hdd_batch_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* params, hdd_p
 
 /* x of one problem: n_pages(problem)*256 floats in the problem's page order. */
 hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host);
+
+/* enable = 1: the preconditioner apply is replayed from a captured HIP graph
+ * (one per state of the smoothing ping-pong buffers); 0 (default): every
+ * kernel is launched from the host (same kernels in the same order, results
+ * bit-identical). Capture needs no other thread of the process to use the
+ * null stream meanwhile. No reference counterpart. */
+hdd_status hdd_set_graph(hdd_ctx* ctx, int enable);
 
 /* Releases all device and host memory of a context. */
 hdd_status hdd_destroy(hdd_ctx* ctx);
@@ -223,6 +236,53 @@ hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out,
  * bottom smooths of SIGMA_MULTIGRID_SOLVER::V_Cycle
  * (IFS/SIGMA_MULTIGRID_SOLVER.cpp:36-47). No reference counterpart. */
 hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max);
+
+/* ---- Serving (no reference counterpart: the reference solves one problem per
+ * process). Requests of different sizes are submitted as they arrive; worker
+ * threads set each one up on the host (as hdd_create) and admit it to the GPU
+ * in groups, each group one batched context (as hdd_create_batch) on its own
+ * stream, several groups side by side, within an HBM budget. A request's
+ * device state stays resident until it has converged; its result is returned
+ * as soon as it stops. Requests in one group need the same vlevels,
+ * sigma_levels and smoothing counts (others wait for a later group). */
+typedef struct hdd_server hdd_server;
+
+typedef struct hdd_server_params {
+    int setup_threads;        /* host setup workers */
+    int groups;               /* groups solved side by side on the device */
+    int max_batch;            /* problems per group */
+    double hbm_budget_bytes;  /* device bytes for resident problems; 0 = 90% of the free device memory */
+} hdd_server_params;
+
+typedef struct hdd_request_report {
+    int iterations;           /* negative if not converged within max_iterations */
+    double final_norm;        /* max|r| at exit */
+    double latency_s;         /* submit -> result */
+    double setup_s;           /* submit -> host setup done (includes waiting for a worker) */
+    double queue_s;           /* host setup done -> admitted to a device group */
+    double solve_s;           /* admitted -> result */
+    int batch_size;           /* problems in its group */
+} hdd_request_report;
+
+hdd_status hdd_server_create(const hdd_server_params* params, hdd_server** out);
+
+/* Queues one problem (copied; the caller's arrays may be freed on return). */
+hdd_status hdd_server_submit(hdd_server* srv, const hdd_problem_desc* desc, const hdd_pcg_params* params, int64_t* request_id);
+
+/* Blocks until the request is done; x (may be NULL) receives n_pages*256
+ * floats in the page order of its desc. Returns HDD_ERR_SETUP / HDD_ERR_DEVICE
+ * with hdd_server_last_error() set if the request failed, HDD_ERR_NOT_CONVERGED
+ * if it stopped at its iteration limit. The request's record is released. */
+hdd_status hdd_server_wait(hdd_server* srv, int64_t request_id, float* x, hdd_request_report* report);
+
+/* Finishes every submitted request, then stops the workers and frees the server. */
+hdd_status hdd_server_destroy(hdd_server* srv);
+
+/* Message of the last failure on this server (or of hdd_server_create when srv is NULL). */
+const char* hdd_server_last_error(const hdd_server* srv);
+
+/* HBM budget of the server in bytes. */
+hdd_status hdd_server_budget(const hdd_server* srv, double* bytes);
 
 #ifdef __cplusplus
 }

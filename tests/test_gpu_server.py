@@ -1,0 +1,94 @@
+"""Serving loop: requests of different sizes submitted one after another, set up
+on host workers, admitted to the GPU in groups that run side by side.
+
+Bar: each request's result in the mix is the result it gets solved alone
+(tolerances as for batched solves: iteration count within +-1, solution
+within 1e-4 relative in max-norm), and every result meets its own tolerance.
+Also: a request larger than the HBM budget is rejected with an error, and
+requests with different V-cycle schedules are solved in separate groups.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from hierarchicaldd_amd import domain
+from tests.gpu_common import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _tol(dom):
+    return 1e-6 * float(np.abs(dom.b).max())
+
+
+def _requests():
+    return [domain.small_box(32, 16, 3, seed=11), domain.small_box(48, 16, 3, seed=12), domain.config0(),
+            domain.small_box(40, 16, 3, seed=13), domain.small_box(32, 32, 3, seed=14), domain.small_box(56, 16, 3, seed=15)]
+
+
+def test_server_results_match_alone():
+    from hierarchicaldd_amd import hdd
+    reqs = _requests()
+    alone = []
+    for dom, cfg in reqs:
+        g = hdd.DDSolver(dom, cfg)
+        it, _, _ = g.solve(_tol(dom), 300)
+        alone.append((it, g.get(hdd.CH_X)))
+        g.close()
+    srv = hdd.Server(setup_threads=2, groups=2, max_batch=3)
+    ids = []
+    for dom, cfg in reqs:
+        ids.append(srv.submit(dom, cfg, _tol(dom), 300))
+        time.sleep(0.02)
+    out = [srv.wait(i) for i in ids]
+    srv.close()
+    for k, ((dom, _), (st, x, rep), (it_a, x_a)) in enumerate(zip(reqs, out, alone)):
+        # the server returns x in the submitted page order; the solve alone in Morton order (same for these domains)
+        print(f"[request {k}] pages={dom.n_pages} alone {it_a} served {rep['iterations']} (group of {rep['batch_size']}), "
+              f"latency {rep['latency_s']:.3f}s setup {rep['setup_s']:.3f}s queue {rep['queue_s']:.3f}s solve {rep['solve_s']:.3f}s, "
+              f"x rel diff {rel_err(x, x_a):.2e}", flush=True)
+        assert st == 0
+        assert abs(rep["iterations"] - it_a) <= 1
+        assert rel_err(x, x_a) < 1e-4
+        assert rep["final_norm"] <= _tol(dom)
+        assert rep["latency_s"] >= rep["solve_s"] >= 0
+
+
+def test_server_input_page_order():
+    """x comes back in the page order of the submitted desc, whatever that order is."""
+    from hierarchicaldd_amd import hdd
+    dom, cfg = domain.small_box(32, 16, 3, seed=21)
+    perm = np.random.default_rng(0).permutation(dom.n_pages)
+    shuffled = domain.SPGridDomain(dom.size, dom.page_offsets[perm].copy(), dom.flags[perm].copy(), dom.b[perm].copy())
+    srv = hdd.Server(setup_threads=1, groups=1, max_batch=2)
+    a = srv.submit(dom, cfg, _tol(dom), 300)
+    b = srv.submit(shuffled, cfg, _tol(dom), 300)
+    _, xa, _ = srv.wait(a)
+    _, xb, _ = srv.wait(b)
+    srv.close()
+    assert rel_err(xb, xa[perm]) < 1e-4
+
+
+def test_server_rejects_oversized():
+    from hierarchicaldd_amd import hdd
+    dom, cfg = domain.config0()
+    srv = hdd.Server(setup_threads=1, groups=1, max_batch=2, hbm_budget=1e6)
+    rid = srv.submit(dom, cfg, _tol(dom), 300)
+    with pytest.raises(hdd.HddError, match="HBM budget"):
+        srv.wait(rid)
+    srv.close()
+
+
+def test_server_mixed_schedules():
+    from hierarchicaldd_amd import hdd
+    a = domain.small_box(32, 16, 3, seed=31)
+    b = domain.small_box(32, 16, 2, seed=32)
+    srv = hdd.Server(setup_threads=2, groups=1, max_batch=4)
+    ia = srv.submit(*a, _tol(a[0]), 300)
+    ib = srv.submit(*b, _tol(b[0]), 300)
+    sa, _, ra = srv.wait(ia)
+    sb, _, rb = srv.wait(ib)
+    srv.close()
+    assert sa == 0 and sb == 0
+    assert ra["batch_size"] == 1 and rb["batch_size"] == 1
