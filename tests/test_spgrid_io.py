@@ -1,0 +1,118 @@
+"""The reference's SPGrid directory format against the only data fixture the
+reference holds: the output of its 2D domain generator
+(demos/SPGrid_Create_Simple_Domain/Create_2D_Domain/output, copied gzip-compressed
+to tests/golden/spgrid2d by tests/golden/make_spgrid2d.py).
+
+* read -> write reproduces the fixture's four files byte for byte;
+* a restatement of the 2D generator's rules (Create_2D_Domain/main.cpp:72-152
+  rasterizer, :217-300 main) reproduces every page offset, flag and value of
+  the fixture: this pins the file layout, the 2D page order (Morton, one bit
+  per axis in turn), the in-page order (last axis fastest), the Dirichlet-ring
+  and obstacle rules, the face-bit order of Flaging and the delta value 4.0.
+  The fixture's delta sits at cell (64, 64) (page 48, element 0), not at the
+  generator's default -position_delta (64, 128): it was made with an explicit
+  position. The geometry is symmetric in x and y, so the fixture does not tell
+  which axis takes the lowest page bit (SPGrid_Mask<.,.,2> gives it to y);
+* 3D domains of the product round-trip through the format unchanged.
+"""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from hierarchicaldd_amd import domain, spgrid_io
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "spgrid2d")
+FILES = ("size", "block_offset", "flags", "data")
+
+
+@pytest.fixture(scope="module")
+def fixture_dir(tmp_path_factory):
+    d = tmp_path_factory.mktemp("spgrid2d")
+    for name in FILES:
+        with gzip.open(os.path.join(GOLDEN, name + ".gz"), "rb") as f:
+            (d / name).write_bytes(f.read())
+    return str(d)
+
+
+def test_roundtrip_bytes(fixture_dir, tmp_path):
+    size, off, fl, data = spgrid_io.read_spgrid(fixture_dir)
+    assert size == (256, 256) and off.shape == (204,) and fl.shape == (204, 256)
+    spgrid_io.write_spgrid(str(tmp_path), size, off, fl, data)
+    for name in FILES:
+        assert (tmp_path / name).read_bytes() == open(os.path.join(fixture_dir, name), "rb").read(), name
+
+
+def _generate_2d(n=256, delta=(64, 64)):
+    """Restatement of the reference 2D generator (float32 arithmetic as its T=float)."""
+    ACTIVE, DIRICHLET, FACE = 0x10000, 0x20000, 0x100000
+    # Rasterizer: grid = GRID(active_domain.max_corner - 1 = n-2 nodes per axis, unit box) -> dX = 1/(n-3)
+    dX = np.float32(1.0) / np.float32(n - 3)
+    idx = np.arange(-1, n + 1)
+    c = (idx.astype(np.float32) - np.float32(0.5)) * dX - np.float32(0.5)
+    cx, cy = c[:, None], c[None, :]
+    inside_bubble = (cx * cx + cy * cy) <= np.float32(0.09)
+    lo, hi = 1, n - 2                                   # interior_domain = active_domain thickened by -1
+    I, J = np.meshgrid(idx, idx, indexing="ij")
+    outside = (I < lo) | (I > hi) | (J < lo) | (J > hi)
+    fl = np.zeros(I.shape, dtype=np.uint32)
+    fl[outside] = DIRICHLET
+    fl[~outside & ~inside_bubble] = ACTIVE
+    # Dirichlet cells without an Active face neighbour are cleared (per block, with a 1-cell halo: same as global here)
+    act = fl == ACTIVE
+    nb = np.zeros_like(act)
+    nb[1:, :] |= act[:-1, :]; nb[:-1, :] |= act[1:, :]; nb[:, 1:] |= act[:, :-1]; nb[:, :-1] |= act[:, 1:]
+    fl[(fl == DIRICHLET) & ~nb] = 0
+    fl = fl[1:-1, 1:-1]                                  # cells 0..n-1
+    # pages of 16 x 16 cells kept iff they hold an Active or Dirichlet cell
+    B = 16
+    blocks = fl.reshape(n // B, B, n // B, B).transpose(0, 2, 1, 3)
+    keep = (blocks != 0).any(axis=(2, 3))
+    # Flaging<2>: face bits (-x,+x,-y,+y) of Active cells whose neighbour is Active|Interface|Dirichlet
+    alloc = np.repeat(np.repeat(keep, B, 0), B, 1)
+    solid = (fl & (ACTIVE | DIRICHLET)) != 0
+    solid &= alloc
+    pad = np.pad(solid, 1)
+    out = fl.copy()
+    for f, nbm in enumerate([pad[0:n, 1:n + 1], pad[2:n + 2, 1:n + 1], pad[1:n + 1, 0:n], pad[1:n + 1, 2:n + 2]]):
+        out[(fl == ACTIVE) & nbm] |= np.uint32(FACE << f)
+    b = np.zeros((n, n), dtype=np.float32)
+    b[delta] = np.float32(4.0)                           # 1/(dx*dx) * 4 with dx = 1
+    bx, by = np.nonzero(keep)
+    key = np.zeros(bx.shape, dtype=np.uint64)
+    for k in range(16):                                  # page index: y in even bits, x in odd bits
+        key |= ((by.astype(np.uint64) >> np.uint64(k)) & np.uint64(1)) << np.uint64(2 * k)
+        key |= ((bx.astype(np.uint64) >> np.uint64(k)) & np.uint64(1)) << np.uint64(2 * k + 1)
+    order = np.argsort(key)
+    bx, by, key = bx[order], by[order], key[order]
+    ob = out.reshape(n // B, B, n // B, B).transpose(0, 2, 1, 3)
+    bb = b.reshape(n // B, B, n // B, B).transpose(0, 2, 1, 3)
+    return key << np.uint64(12), ob[bx, by].reshape(-1, 256), bb[bx, by].reshape(-1, 256)
+
+
+def test_generator_rules_reproduce_fixture(fixture_dir):
+    size, off, fl, data = spgrid_io.read_spgrid(fixture_dir)
+    goff, gfl, gdata = _generate_2d()
+    assert np.array_equal(goff, off)
+    mism = np.argwhere(gfl != fl)
+    assert mism.size == 0, f"{len(mism)} flag mismatches, first {mism[:5]}"
+    assert np.array_equal(gdata.view(np.uint32), data.view(np.uint32))
+
+
+def test_3d_domain_roundtrip(tmp_path):
+    dom, cfg = domain.config0()
+    spgrid_io.write_domain(str(tmp_path), dom)
+    size, _, _, _ = spgrid_io.read_spgrid(str(tmp_path))
+    assert size == (64, 64, 64)
+    back = spgrid_io.read_domain(str(tmp_path), dom.size)
+    assert np.array_equal(back.page_offsets, dom.page_offsets)
+    assert np.array_equal(back.flags, dom.flags)
+    assert np.array_equal(back.b.view(np.uint32), dom.b.view(np.uint32))
+
+
+def test_padded_size_rule():
+    # SPGrid_Geometry<3>: power-of-two padding with the z extent shared by all axes' lower bounds
+    assert spgrid_io.padded_size_3d(256, 256, 256) == (256, 256, 256)
+    assert spgrid_io.padded_size_3d(100, 100, 100) == (128, 128, 128)
+    assert spgrid_io.padded_size_3d(16, 16, 128) == (64, 64, 128)
