@@ -13,6 +13,9 @@ on the context stream; cpu_baseline is the oracle (the CPU restatement of the
 reference dd_pcg) on a bounded sample: the complete 64^3 problem (setup +
 solve), set beside the GPU's rate on the same 64^3 problem.
 
+"fp16_storage" repeats the 256^3 solve with the V-cycle vectors stored in FP16
+(FP32 arithmetic) and reports its time, iterations and level-0 sweep rate.
+
 The other two parts of the metric are added to the same line:
   "batched": uneven independent problems (edge 64-256, 1-4 sphere obstacles,
       delta right-hand side, sub_size 32; --batch-count of them, 16 by default,
@@ -107,6 +110,21 @@ def main():
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic, {achieved:.0f} GB/s")
 
+    # FP16-storage V-cycles (FP32 arithmetic) on the same problem
+    solver.set_fp16(True)
+    solver.solve(tol, args.max_iterations)
+    t16, it16 = [], []
+    for k in range(max(1, min(args.steps, 2))):
+        it, hist, secs = solver.solve(tol, args.max_iterations)
+        t16.append(secs)
+        it16.append(it)
+    h_ms, h_bytes = solver.profile_sweep(100)
+    solver.set_fp16(False)
+    fp16 = {"time_to_tolerance_s": float(np.mean(t16)), "solves_per_s": 1.0 / float(np.mean(t16)), "iterations": int(np.median(it16)),
+            "sweep_us": h_ms * 1e3, "sweep_achieved_GBs": h_bytes / (h_ms * 1e-3) / 1e9,
+            "sweep_frac": h_bytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "sweep_bytes_per_slot": 10}
+    log(f"[bench] fp16-storage V-cycles: {fp16['iterations']} iterations, {fp16['time_to_tolerance_s']:.3f}s; "
+        f"level-0 sweep {h_ms * 1e3:.1f} us ({fp16['sweep_achieved_GBs']:.0f} GB/s algorithmic at 10 B/slot)")
     solver.close()
     cpu = None
     if not args.no_cpu_baseline:
@@ -136,6 +154,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": counter_traffic(dom.n_pages) if args.config == "sphere256" else None},
         "cpu_baseline": cpu,
+        "fp16_storage": fp16,
         "batched": batched,
         "served": served,
     }

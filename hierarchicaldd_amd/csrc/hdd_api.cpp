@@ -170,6 +170,12 @@ hdd_status hdd_set_graph(hdd_ctx* ctx, int enable) {
     return HDD_OK;
 }
 
+hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->set_half_storage(fp16 != 0));
+}
+
 hdd_status hdd_destroy(hdd_ctx* ctx) {
     if (!ctx) return HDD_ERR_ARG;
     delete ctx;
@@ -339,7 +345,7 @@ hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* byt
         hdd_check(hipEventElapsedTime(&ms, a, b));
         (void)hipEventDestroy(a); (void)hipEventDestroy(b);
         *avg_ms = ms / reps;
-        if (bytes) *bytes = 16.0 * kPageCells * S.lv[0].host.npages();
+        if (bytes) *bytes = (S.half_storage ? 10.0 : 16.0) * kPageCells * S.lv[0].host.npages();
     })
 }
 

@@ -1,6 +1,7 @@
 // Device-side views shared by the HIP kernels and the host runtime.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -31,12 +32,21 @@ struct DevLevel {
 };
 
 void upload_constants();
+// V-cycle kernels, float storage and half (FP16) storage with float arithmetic
 void launch_smooth(const DevLevel& L, const float* u_in, float* u_out, const float* b, bool interior, bool boundary, hipStream_t s);
+void launch_smooth(const DevLevel& L, const __half* u_in, __half* u_out, const __half* b, bool interior, bool boundary, hipStream_t s);
 void launch_residual(const DevLevel& L, const float* u, const float* b, float* r, hipStream_t s);
+void launch_residual(const DevLevel& L, const __half* u, const __half* b, __half* r, hipStream_t s);
 void launch_fill(float* a, float v, size_t n, hipStream_t s);
+void launch_fill(__half* a, float v, size_t n, hipStream_t s);
 void launch_copy(float* d, const float* src, size_t n, hipStream_t s);
 void launch_restrict(const DevLevel& F, const DevLevel& C, const float* r_f, float* b_c, hipStream_t s);
+void launch_restrict(const DevLevel& F, const DevLevel& C, const __half* r_f, __half* b_c, hipStream_t s);
 void launch_prolong(const DevLevel& F, const DevLevel& C, float* u_f, const float* u_c, hipStream_t s);
+void launch_prolong(const DevLevel& F, const DevLevel& C, __half* u_f, const __half* u_c, hipStream_t s);
+// FP16 storage: hb = r / max|r| per problem (scalar slot 4), and z = max|r| * hz projected
+void launch_scale_to_half(const float* r, __half* hb, const int* page_prob, const float* scal, size_t n, hipStream_t s);
+void launch_unscale_project(const DevLevel& L, const __half* hz, float* z, const int* page_prob, const float* scal, hipStream_t s);
 void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s);
 void launch_project(const DevLevel& L, float* v, hipStream_t s);
 void launch_sub_project(const DevLevel& L, const float* b, const float* q, float* r, hipStream_t s);

@@ -16,6 +16,7 @@
 //                                           LIB/SPGrid_Prolongation_Legacy.h:48-123
 //   PCG Laplace (double accumulation)       DDS/SPGrid_Gemini_Arithmetic_Helpers.h:240-262
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <cstdint>
 #include <mutex>
 #include "hdd_device.h"
@@ -50,16 +51,24 @@ __device__ __forceinline__ int nb_index(const int* __restrict__ nbr, int p, int 
     }
 }
 
+// V-cycle storage: float, or half (FP16 storage, FP32 arithmetic: values are
+// widened on load and rounded to nearest on store).
+__device__ __forceinline__ float ld(const float* __restrict__ p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ld(const __half* __restrict__ p, size_t i) { return __half2float(p[i]); }
+__device__ __forceinline__ void st(float* __restrict__ p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void st(__half* __restrict__ p, size_t i, float v) { p[i] = __float2half_rn(v); }
+
 // float residual of one cell, reference operation order. The six neighbour
 // values are loaded unconditionally (the zero page backs missing pages, so
 // every address is valid) and selected by the face bits afterwards: the loads
 // then leave together with the cell's own, instead of waiting for its flags.
+template <class S>
 __device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int p, int c, uint32_t f,
-                                               const float* __restrict__ u, float b) {
-    const float cv = u[p * 256 + c];
+                                               const S* __restrict__ u, float b) {
+    const float cv = ld(u, p * 256 + c);
     float nv[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) nv[k] = u[nb_index(nbr, p, c, k)];
+    for (int k = 0; k < 6; ++k) nv[k] = ld(u, nb_index(nbr, p, c, k));
     float result = 0.f;
 #pragma unroll
     for (int k = 0; k < 6; ++k)
@@ -69,30 +78,33 @@ __device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int 
 
 // One damped-Jacobi sweep (residual + correction fused, ping-pong buffers).
 // Cells outside the selection are copied unchanged.
-__global__ void __launch_bounds__(256) k_smooth(DevLevel L, const float* __restrict__ u_in, float* __restrict__ u_out,
-                                                const float* __restrict__ b, int write_interior, int write_boundary) {
+template <class S>
+__global__ void __launch_bounds__(256) k_smooth(DevLevel L, const S* __restrict__ u_in, S* __restrict__ u_out,
+                                                const S* __restrict__ b, int write_interior, int write_boundary) {
     const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= L.np || page_stopped(L, p)) return;
     const uint32_t f = L.flags[e];
     const uint32_t sel = f & (kActive | kBoundary);
     const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
-    float u = u_in[e];
-    const float r = cell_residual(L.nbr, p, c, f, u_in, b[e]);
+    float u = ld(u_in, e);
+    const float r = cell_residual(L.nbr, p, c, f, u_in, ld(b, e));
     if (w) u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * r;
-    u_out[e] = u;
+    st(u_out, e, u);
 }
 
-__global__ void __launch_bounds__(256) k_residual(DevLevel L, const float* __restrict__ u, const float* __restrict__ b,
-                                                  float* __restrict__ r) {
+template <class S>
+__global__ void __launch_bounds__(256) k_residual(DevLevel L, const S* __restrict__ u, const S* __restrict__ b,
+                                                  S* __restrict__ r) {
     const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= L.np || page_stopped(L, p)) return;
     const uint32_t f = L.flags[e];
-    if (f & kActive) r[e] = cell_residual(L.nbr, p, c, f, u, b[e]);
+    if (f & kActive) st(r, e, cell_residual(L.nbr, p, c, f, u, ld(b, e)));
 }
 
-__global__ void k_fill(float* __restrict__ a, float v, size_t n) {
+template <class S>
+__global__ void k_fill(S* __restrict__ a, float v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (size_t)gridDim.x * blockDim.x) a[i] = v;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) st(a, i, v);
 }
 __global__ void k_copy(float* __restrict__ d, const float* __restrict__ s, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,7 +119,8 @@ __device__ __forceinline__ int lut_page(const DevLevel& L, int x, int y, int z) 
 }
 
 // coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only)
-__global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const float* __restrict__ r_f, float* __restrict__ b_c) {
+template <class S>
+__global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const S* __restrict__ r_f, S* __restrict__ b_c) {
     const int p = xcd_page(C.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= C.np || page_stopped(C, p)) return;
     if (!(C.flags[e] & kActive)) return;
@@ -119,28 +132,29 @@ __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const 
             for (int k = 0; k < 4; ++k) {
                 const int X = fx + i, Y = fy + j, Z = fz + k;
                 const int q = lut_page(F, X, Y, Z);
-                acc += c_rweights[i * 16 + j * 4 + k] * r_f[q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7))];
+                acc += c_rweights[i * 16 + j * 4 + k] * ld(r_f, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
             }
-    b_c[e] = acc;
+    st(b_c, e, acc);
 }
 
 // fine u += sum_{2x2x2} w u_coarse (Active fine cells only)
-__global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, float* __restrict__ u_f, const float* __restrict__ u_c) {
+template <class S>
+__global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, S* __restrict__ u_f, const S* __restrict__ u_c) {
     const int p = xcd_page(F.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= F.np || page_stopped(F, p)) return;
     if (!(F.flags[e] & kActive)) return;
     const int x = F.base[p * 3 + 0] + (c >> 6), y = F.base[p * 3 + 1] + ((c >> 3) & 7), z = F.base[p * 3 + 2] + (c & 7);
     const int cx = (x + 1) >> 1, cy = (y + 1) >> 1, cz = (z + 1) >> 1;
     const int px = 1 - (x - (2 * cx - 1)), py = 1 - (y - (2 * cy - 1)), pz = 1 - (z - (2 * cz - 1));
-    float u = u_f[e];
+    float u = ld(u_f, e);
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int k = 0; k < 2; ++k) {
                 const int X = cx + i - px, Y = cy + j - py, Z = cz + k - pz;
                 const int q = lut_page(C, X, Y, Z);
-                u += c_pweights[((i ^ px) << 2) | ((j ^ py) << 1) | (k ^ pz)] * u_c[q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7))];
+                u += c_pweights[((i ^ px) << 2) | ((j ^ py) << 1) | (k ^ pz)] * ld(u_c, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
             }
-    u_f[e] = u;
+    st(u_f, e, u);
 }
 
 // PCG operator: out = L v on Active|Interface cells (double accumulation), 0 elsewhere (fused Project)
@@ -279,6 +293,28 @@ __global__ void __launch_bounds__(256) k_seg_finish(const double* __restrict__ d
         }
     }
 }
+// FP16 storage: the preconditioner is linear, so it runs on r / a and its
+// result is scaled back by a, with a = max|r| of the cell's problem (slot 4 of
+// its scalars; 1 where r = 0). Values then stay O(1), inside half's normal range.
+__global__ void __launch_bounds__(256) k_scale_to_half(const float* __restrict__ r, __half* __restrict__ hb, const int* __restrict__ page_prob,
+                                                       const float* __restrict__ scal, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int pr = page_prob[i >> 8];
+        if (pr < 0) continue;
+        const float a = scal[pr * 8 + 4];
+        hb[i] = __float2half_rn(a > 0.f ? r[i] / a : 0.f);
+    }
+}
+// z = a * z16 on Active|Interface cells, 0 elsewhere (Project)
+__global__ void __launch_bounds__(256) k_unscale_project(DevLevel L, const __half* __restrict__ hz, float* __restrict__ z,
+                                                         const int* __restrict__ page_prob, const float* __restrict__ scal) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int pr = page_prob[e >> 8];
+    const float a = pr >= 0 ? scal[pr * 8 + 4] : 0.f;
+    z[e] = (pr >= 0 && (L.flags[e] & (kActive | kInterface))) ? a * __half2float(hz[e]) : 0.f;
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 static inline int grid_for(size_t n) { size_t g = (n + 255) / 256; return (int)(g > 4096 ? 4096 : (g == 0 ? 1 : g)); }
@@ -305,23 +341,48 @@ void upload_constants() {
     std::call_once(once, upload_constants_once);
 }
 
+template <class S>
+static void smooth_t(const DevLevel& L, const S* u_in, S* u_out, const S* b, bool interior, bool boundary, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_smooth<S>, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u_in, u_out, b, (int)interior, (int)boundary);
+}
+template <class S>
+static void residual_t(const DevLevel& L, const S* u, const S* b, S* r, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_residual<S>, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u, b, r);
+}
+template <class S>
+static void fill_t(S* a, float v, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill<S>, dim3(grid_for(n)), dim3(256), 0, s, a, v, n);
+}
+template <class S>
+static void restrict_t(const DevLevel& F, const DevLevel& C, const S* r_f, S* b_c, hipStream_t s) {
+    if (C.np) hipLaunchKernelGGL(k_restrict<S>, dim3(kXcds * xcd_chunk(C.np)), dim3(256), 0, s, F, C, r_f, b_c);
+}
+template <class S>
+static void prolong_t(const DevLevel& F, const DevLevel& C, S* u_f, const S* u_c, hipStream_t s) {
+    if (F.np) hipLaunchKernelGGL(k_prolong<S>, dim3(kXcds * xcd_chunk(F.np)), dim3(256), 0, s, F, C, u_f, u_c);
+}
 void launch_smooth(const DevLevel& L, const float* u_in, float* u_out, const float* b, bool interior, bool boundary, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_smooth, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u_in, u_out, b, (int)interior, (int)boundary);
+    smooth_t(L, u_in, u_out, b, interior, boundary, s);
 }
-void launch_residual(const DevLevel& L, const float* u, const float* b, float* r, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_residual, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u, b, r);
+void launch_smooth(const DevLevel& L, const __half* u_in, __half* u_out, const __half* b, bool interior, bool boundary, hipStream_t s) {
+    smooth_t(L, u_in, u_out, b, interior, boundary, s);
 }
-void launch_fill(float* a, float v, size_t n, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, a, v, n);
-}
+void launch_residual(const DevLevel& L, const float* u, const float* b, float* r, hipStream_t s) { residual_t(L, u, b, r, s); }
+void launch_residual(const DevLevel& L, const __half* u, const __half* b, __half* r, hipStream_t s) { residual_t(L, u, b, r, s); }
+void launch_fill(float* a, float v, size_t n, hipStream_t s) { fill_t(a, v, n, s); }
+void launch_fill(__half* a, float v, size_t n, hipStream_t s) { fill_t(a, v, n, s); }
 void launch_copy(float* d, const float* src, size_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(256), 0, s, d, src, n);
 }
-void launch_restrict(const DevLevel& F, const DevLevel& C, const float* r_f, float* b_c, hipStream_t s) {
-    if (C.np) hipLaunchKernelGGL(k_restrict, dim3(kXcds * xcd_chunk(C.np)), dim3(256), 0, s, F, C, r_f, b_c);
+void launch_restrict(const DevLevel& F, const DevLevel& C, const float* r_f, float* b_c, hipStream_t s) { restrict_t(F, C, r_f, b_c, s); }
+void launch_restrict(const DevLevel& F, const DevLevel& C, const __half* r_f, __half* b_c, hipStream_t s) { restrict_t(F, C, r_f, b_c, s); }
+void launch_prolong(const DevLevel& F, const DevLevel& C, float* u_f, const float* u_c, hipStream_t s) { prolong_t(F, C, u_f, u_c, s); }
+void launch_prolong(const DevLevel& F, const DevLevel& C, __half* u_f, const __half* u_c, hipStream_t s) { prolong_t(F, C, u_f, u_c, s); }
+void launch_scale_to_half(const float* r, __half* hb, const int* page_prob, const float* scal, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_scale_to_half, dim3(grid_for(n)), dim3(256), 0, s, r, hb, page_prob, scal, n);
 }
-void launch_prolong(const DevLevel& F, const DevLevel& C, float* u_f, const float* u_c, hipStream_t s) {
-    if (F.np) hipLaunchKernelGGL(k_prolong, dim3(kXcds * xcd_chunk(F.np)), dim3(256), 0, s, F, C, u_f, u_c);
+void launch_unscale_project(const DevLevel& L, const __half* hz, float* z, const int* page_prob, const float* scal, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_unscale_project, dim3(L.np + 1), dim3(256), 0, s, L, hz, z, page_prob, scal);
 }
 void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s) {
     if (L.np) hipLaunchKernelGGL(k_laplace_pcg, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, v, out);
@@ -359,8 +420,9 @@ namespace hdd {
 // b0[i] = r_e - (float)(-sum over Active face neighbours of z, double accumulation)
 // for each interface dof i at level-0 cell e (Laplace_ri + Copy_To_Eigen_Array,
 // LIB/SPGrid_Laplace_Legacy.h:95-116, IFS/EIGEN_INTERFACE.cpp:18-30)
-__global__ void k_if_gather(DevLevel L, const int* __restrict__ if_index, int n, const float* __restrict__ z,
-                            const float* __restrict__ r, float* __restrict__ b0) {
+template <class S>
+__global__ void k_if_gather(DevLevel L, const int* __restrict__ if_index, int n, const S* __restrict__ z,
+                            const S* __restrict__ r, float* __restrict__ b0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int e = if_index[i];
@@ -372,23 +434,30 @@ __global__ void k_if_gather(DevLevel L, const int* __restrict__ if_index, int n,
     for (int k = 0; k < 6; ++k)
         if (f & (kFaceMX << k)) {
             const int q = nb_index(L.nbr, p, c, k);
-            if (L.flags[q] & kActive) res -= (double)z[q];
+            if (L.flags[q] & kActive) res -= (double)ld(z, q);
         }
-    b0[i] = r[e] - (float)res;
+    b0[i] = ld(r, e) - (float)res;
 }
 
-__global__ void k_if_scatter(const int* __restrict__ if_index, int n, const float* __restrict__ u0, float* __restrict__ z) {
+template <class S>
+__global__ void k_if_scatter(const int* __restrict__ if_index, int n, const float* __restrict__ u0, S* __restrict__ z) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int e = if_index[i];
-    if (e >= 0) z[e] = u0[i];
+    if (e >= 0) st(z, e, u0[i]);
 }
 
 void launch_if_gather(const DevLevel& L0, const int* if_index, int n, const float* z, const float* r, float* b0, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_if_gather, dim3((n + 255) / 256), dim3(256), 0, s, L0, if_index, n, z, r, b0);
+    if (n) hipLaunchKernelGGL(k_if_gather<float>, dim3((n + 255) / 256), dim3(256), 0, s, L0, if_index, n, z, r, b0);
+}
+void launch_if_gather(const DevLevel& L0, const int* if_index, int n, const __half* z, const __half* r, float* b0, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_if_gather<__half>, dim3((n + 255) / 256), dim3(256), 0, s, L0, if_index, n, z, r, b0);
 }
 void launch_if_scatter(const int* if_index, int n, const float* u0, float* z, hipStream_t s) {
-    if (n) hipLaunchKernelGGL(k_if_scatter, dim3((n + 255) / 256), dim3(256), 0, s, if_index, n, u0, z);
+    if (n) hipLaunchKernelGGL(k_if_scatter<float>, dim3((n + 255) / 256), dim3(256), 0, s, if_index, n, u0, z);
+}
+void launch_if_scatter(const int* if_index, int n, const float* u0, __half* z, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_if_scatter<__half>, dim3((n + 255) / 256), dim3(256), 0, s, if_index, n, u0, z);
 }
 
 }  // namespace hdd

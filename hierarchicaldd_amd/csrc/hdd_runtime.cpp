@@ -245,7 +245,7 @@ void Solver::upload_levels() {
 float* Solver::channel(int c) {
     LevelBuffers& L0 = lv[0];
     switch (c) {
-        case 0: return L0.u[L0.cur].p;
+        case 0: return half_storage ? zf.p : L0.u[L0.cur].p;
         case 1: return L0.b.p;
         case 2: return L0.r.p;
         case 3: return x.p;
@@ -256,37 +256,82 @@ float* Solver::channel(int c) {
     throw std::runtime_error("bad channel id");
 }
 
-void Solver::sweeps(int l, int count, bool interior, bool boundary) {
+namespace {
+// V-cycle buffers of one level by storage type
+template <class S> struct VStore;
+template <> struct VStore<float> {
+    static float* u(LevelBuffers& L, int i) { return L.u[i].p; }
+    static float* b(LevelBuffers& L) { return L.b.p; }
+    static float* r(LevelBuffers& L) { return L.r.p; }
+};
+template <> struct VStore<__half> {
+    static __half* u(LevelBuffers& L, int i) { return L.hu[i].p; }
+    static __half* b(LevelBuffers& L) { return L.hb.p; }
+    static __half* r(LevelBuffers& L) { return L.hr.p; }
+};
+}  // namespace
+
+template <class S>
+void Solver::sweeps_t(int l, int count, bool interior, bool boundary) {
     LevelBuffers& L = lv[l];
     const DevLevel v = L.view();
     for (int i = 0; i < count; ++i) {
-        launch_smooth(v, L.u[L.cur].p, L.u[1 - L.cur].p, L.b.p, interior, boundary, stream);
+        launch_smooth(v, VStore<S>::u(L, L.cur), VStore<S>::u(L, 1 - L.cur), VStore<S>::b(L), interior, boundary, stream);
         L.cur = 1 - L.cur;
     }
 }
 
-void Solver::smooth0(int mode) { sweeps(0, 1, mode == 0 || mode == 2, mode == 1 || mode == 2); }
+void Solver::sweeps(int l, int count, bool interior, bool boundary) { sweeps_t<float>(l, count, interior, boundary); }
+
+void Solver::smooth0(int mode) {
+    const bool in = mode == 0 || mode == 2, bd = mode == 1 || mode == 2;
+    if (half_storage) sweeps_t<__half>(0, 1, in, bd);
+    else sweeps(0, 1, in, bd);
+}
 
 // V_Cycle_Helper::v_cycle (LIB/SPGrid_V_Cycle_Helper.h:156-192). The two
 // bottom-level residual passes only write the residual channel, which nothing
 // reads before it is recomputed, so they are not launched.
-void Solver::vcycle() {
+template <class S>
+void Solver::vcycle_t() {
+    using V = VStore<S>;
     const int L = (int)lv.size();
     for (int l = 0; l < L - 1; ++l) {
-        sweeps(l, P.bnd_s, false, true);
-        sweeps(l, P.int_s, true, false);
-        sweeps(l, P.bnd_s, false, true);
-        launch_residual(lv[l].view(), lv[l].u[lv[l].cur].p, lv[l].b.p, lv[l].r.p, stream);
-        launch_fill(lv[l + 1].u[lv[l + 1].cur].p, 0.f, lv[l + 1].cells(), stream);
-        launch_restrict(lv[l].view(), lv[l + 1].view(), lv[l].r.p, lv[l + 1].b.p, stream);
+        sweeps_t<S>(l, P.bnd_s, false, true);
+        sweeps_t<S>(l, P.int_s, true, false);
+        sweeps_t<S>(l, P.bnd_s, false, true);
+        launch_residual(lv[l].view(), V::u(lv[l], lv[l].cur), V::b(lv[l]), V::r(lv[l]), stream);
+        launch_fill(V::u(lv[l + 1], lv[l + 1].cur), 0.f, lv[l + 1].cells(), stream);
+        launch_restrict(lv[l].view(), lv[l + 1].view(), V::r(lv[l]), V::b(lv[l + 1]), stream);
     }
-    sweeps(L - 1, 200, true, true);
+    sweeps_t<S>(L - 1, 200, true, true);
     for (int l = L - 2; l >= 0; --l) {
-        launch_prolong(lv[l].view(), lv[l + 1].view(), lv[l].u[lv[l].cur].p, lv[l + 1].u[lv[l + 1].cur].p, stream);
-        sweeps(l, P.bnd_s, false, true);
-        sweeps(l, P.int_s, true, false);
-        sweeps(l, P.bnd_s, false, true);
+        launch_prolong(lv[l].view(), lv[l + 1].view(), V::u(lv[l], lv[l].cur), V::u(lv[l + 1], lv[l + 1].cur), stream);
+        sweeps_t<S>(l, P.bnd_s, false, true);
+        sweeps_t<S>(l, P.int_s, true, false);
+        sweeps_t<S>(l, P.bnd_s, false, true);
     }
+}
+
+void Solver::vcycle() { vcycle_t<float>(); }
+
+void Solver::set_half_storage(bool on) {
+    if (on && !zf.p) {
+        for (auto& L : lv) {
+            L.hu[0].alloc(L.cells()); L.hu[1].alloc(L.cells());
+            L.hb.alloc(L.cells()); L.hr.alloc(L.cells());
+        }
+        zf.alloc(lv[0].cells());
+    }
+    half_storage = on;
+}
+
+DevSegments Solver::segments() {
+    DevSegments sg;
+    sg.nprob = nprob(); sg.max_parts = max_parts; sg.ncells = (size_t)lv[0].host.npages() * kPageCells;
+    sg.pg0 = seg_pg0.p; sg.parts = seg_parts.p; sg.page_prob = page_prob.p; sg.active = active.p;
+    sg.dpart = dpart.p; sg.fpart = fpart.p; sg.scal = scal.p; sg.dscal = dscal.p;
+    return sg;
 }
 
 // Apply_Preconditioner (DDS/SPGrid_Domain_Decomposition_Preconditioner.h:88-128),
@@ -305,6 +350,7 @@ void Solver::apply_preconditioner() {
     std::vector<int> key;
     for (auto& L : lv) key.push_back(L.cur);
     key.push_back(lv[0].active != nullptr);
+    key.push_back(half_storage);
     for (auto& g : graphs)
         if (g.key == key) {
             hdd_check(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
@@ -336,6 +382,21 @@ void Solver::apply_preconditioner() {
 void Solver::record_apply() {
     LevelBuffers& L0 = lv[0];
     const size_t n0 = L0.cells();
+    if (half_storage) {
+        // FP16 storage: the V-cycles and the interface exchange run on r / a per
+        // problem (a = max|r|, recomputed here), the interface solver in FP32
+        launch_seg_maxabs(segments(), L0.b.p, stream);
+        launch_scale_to_half(L0.b.p, L0.hb.p, page_prob.p, scal.p, (size_t)L0.host.npages() * kPageCells, stream);
+        launch_fill(L0.hu[L0.cur].p, 0.f, n0, stream);
+        for (int i = 0; i < P.nV; ++i) vcycle_t<__half>();
+        launch_interface_gather(*sigma, L0.view(), if_index.p, L0.hu[L0.cur].p, L0.hb.p, stream);
+        sigma_vcycles(*sigma, stream);
+        launch_fill(L0.hu[L0.cur].p, 0.f, n0, stream);
+        launch_interface_scatter(*sigma, if_index.p, L0.hu[L0.cur].p, stream);
+        for (int i = 0; i < P.nV; ++i) vcycle_t<__half>();
+        launch_unscale_project(L0.view(), L0.hu[L0.cur].p, zf.p, page_prob.p, scal.p, stream);
+        return;
+    }
     launch_fill(L0.u[L0.cur].p, 0.f, n0, stream);
     for (int i = 0; i < P.nV; ++i) vcycle();
     interface_gather();
@@ -367,10 +428,7 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
     const DevLevel v0 = L0.view();
     const size_t n0 = L0.cells();
     float* r = L0.b.p;
-    DevSegments sg;
-    sg.nprob = NP; sg.max_parts = max_parts; sg.ncells = (size_t)L0.host.npages() * kPageCells;
-    sg.pg0 = seg_pg0.p; sg.parts = seg_parts.p; sg.page_prob = page_prob.p; sg.active = active.p;
-    sg.dpart = dpart.p; sg.fpart = fpart.p; sg.scal = scal.p; sg.dscal = dscal.p;
+    const DevSegments sg = segments();
     out.assign(NP, PcgResult());
     const float flt_max = 3.40282347e+38f;
     std::vector<float> hs((size_t)NP * 8, 0.f);
@@ -426,7 +484,7 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
         // act is not modified again before the next synchronisation
         if (changed) hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
         apply_preconditioner();
-        float* z = L0.u[L0.cur].p;
+        float* z = channel(0);
         launch_seg_dot(sg, z, r, 0, stream);                   // rho, beta
         if (restart) launch_copy(s.p, z, n0, stream);
         else launch_seg_copy2(sg, s.p, s.p, z, 2, stream);     // s = s*beta + z

@@ -95,6 +95,7 @@ struct LevelBuffers {
     DBuf<int> prob;                 // problem of each page (zero page: -1)
     const int* active = nullptr;    // set while a batch PCG runs: its per-problem flags
     DBuf<float> u[2], b, r;
+    DBuf<__half> hu[2], hb, hr;     // FP16-storage variant of u, b, r (allocated when it is selected)
     int cur = 0;  // which u buffer holds the current iterate
     DevLevel view() const;
     size_t cells() const { return (size_t)(host.npages() + 1) * kPageCells; }
@@ -124,6 +125,8 @@ public:
     double setup_seconds = 0;
     bool host_only = false;
     bool use_graph = false;            // replay the preconditioner apply from captured HIP graphs (opt-in)
+    bool half_storage = false;         // V-cycles store u, b, r in FP16 (FP32 arithmetic); z then lives in zf
+    DBuf<float> zf;                    // preconditioner output of the FP16-storage variant
 
     // one problem; host_only: run the host setup and skip every device allocation (introspection)
     Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b,
@@ -147,6 +150,9 @@ public:
                    const std::function<void(int)>& on_stop = nullptr);
     // x of problem p (its level-0 pages in its own page order), n_pages(p)*256 floats
     void download_x(int p, float* host);
+    // V-cycle storage of the preconditioner: FP32 (the reference's) or FP16 with FP32 arithmetic
+    void set_half_storage(bool on);
+    DevSegments segments();
 
 private:
     struct Graph {
@@ -160,6 +166,8 @@ private:
     void merge_levels();
     void upload_levels();
     void sweeps(int l, int count, bool interior, bool boundary);
+    template <class S> void sweeps_t(int l, int count, bool interior, bool boundary);
+    template <class S> void vcycle_t();
     void interface_gather();
     void interface_scatter();
 };

@@ -115,12 +115,16 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
                                                int smoothing, int n_mg, hipStream_t stream);
 void sigma_vcycles(SigmaDevice& S, hipStream_t s);
 void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s);
+void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const __half* z, const __half* r, hipStream_t s);
 void launch_interface_scatter(SigmaDevice& S, const int* if_index, float* z, hipStream_t s);
+void launch_interface_scatter(SigmaDevice& S, const int* if_index, __half* z, hipStream_t s);
 void sigma_vcycle_host(SigmaDevice& S, const float* b, float* u, hipStream_t s);
 void sigma_smooth_host(SigmaDevice& S, int level, const float* b, float* u, hipStream_t s);
 
 // interface gather (Laplace_ri + Copy_To_Eigen_Array) and scatter (Copy_From_Eigen_Array), in hdd_kernels.hip
 void launch_if_gather(const DevLevel& L0, const int* if_index, int n, const float* z, const float* r, float* b0, hipStream_t s);
+void launch_if_gather(const DevLevel& L0, const int* if_index, int n, const __half* z, const __half* r, float* b0, hipStream_t s);
 void launch_if_scatter(const int* if_index, int n, const float* u0, float* z, hipStream_t s);
+void launch_if_scatter(const int* if_index, int n, const float* u0, __half* z, hipStream_t s);
 
 }  // namespace hdd

@@ -881,15 +881,24 @@ void sigma_vcycles(SigmaDevice& S, hipStream_t s) {
     for (int i = 0; i < S.n_mg; ++i) S.vcycle(s, i == 0);
 }
 
-void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s) {
+template <class S_>
+static void gather_t(SigmaDevice& S, const DevLevel& L0, const int* if_index, const S_* z, const S_* r, hipStream_t s) {
     if (S.levels == 0 || S.lv[0].n_if == 0) return;
     launch_if_gather(L0, if_index, S.lv[0].n_if, z, r, S.b0, s);
 }
-
-void launch_interface_scatter(SigmaDevice& S, const int* if_index, float* z, hipStream_t s) {
+template <class S_>
+static void scatter_t(SigmaDevice& S, const int* if_index, S_* z, hipStream_t s) {
     if (S.levels == 0 || S.lv[0].n_if == 0) return;
     launch_if_scatter(if_index, S.lv[0].n_if, S.u0, z, s);
 }
+void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const float* z, const float* r, hipStream_t s) {
+    gather_t(S, L0, if_index, z, r, s);
+}
+void launch_interface_gather(SigmaDevice& S, const DevLevel& L0, const int* if_index, const __half* z, const __half* r, hipStream_t s) {
+    gather_t(S, L0, if_index, z, r, s);
+}
+void launch_interface_scatter(SigmaDevice& S, const int* if_index, float* z, hipStream_t s) { scatter_t(S, if_index, z, s); }
+void launch_interface_scatter(SigmaDevice& S, const int* if_index, __half* z, hipStream_t s) { scatter_t(S, if_index, z, s); }
 
 void sigma_vcycle_host(SigmaDevice& S, const float* b, float* u, hipStream_t s) {
     if (S.levels == 0) throw std::runtime_error("no sigma levels");

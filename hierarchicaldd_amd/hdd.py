@@ -54,7 +54,7 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats",
            "hdd_profile_solve", "hdd_create_batch", "hdd_batch_count", "hdd_batch_num_pages", "hdd_batch_page_offsets",
            "hdd_batch_pcg_solve", "hdd_batch_get_x", "hdd_create_batch_from", "hdd_server_create", "hdd_server_submit",
-           "hdd_server_wait", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph"]
+           "hdd_server_wait", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph", "hdd_set_vcycle_storage"]
 
 
 def lib():
@@ -108,6 +108,7 @@ def lib():
         L.hdd_server_last_error.restype = C.c_char_p
         L.hdd_server_budget.argtypes = [P, C.POINTER(C.c_double)]
         L.hdd_set_graph.argtypes = [P, I]
+        L.hdd_set_vcycle_storage.argtypes = [P, I]
         for name in EXPORTS:
             if name not in ("hdd_last_error", "hdd_server_last_error"):
                 getattr(L, name).restype = I
@@ -245,6 +246,10 @@ class DDSolver:
     def set_graph(self, enable: bool):
         """Replay the preconditioner apply from captured HIP graphs, or launch every kernel (default)."""
         self._check(lib().hdd_set_graph(self.h, int(bool(enable))))
+
+    def set_fp16(self, enable: bool):
+        """FP16 storage (FP32 arithmetic) for the V-cycle vectors of the preconditioner."""
+        self._check(lib().hdd_set_vcycle_storage(self.h, int(bool(enable))))
 
     def solve(self, tolerance: float, max_iterations: int = 5000):
         """PCG from x=0; returns (iterations, residual history, seconds)."""

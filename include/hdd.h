@@ -117,6 +117,17 @@ hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host);
  * null stream meanwhile. No reference counterpart. */
 hdd_status hdd_set_graph(hdd_ctx* ctx, int enable);
 
+/* V-cycle storage of the preconditioner (no reference counterpart; the
+ * reference stores everything in float): fp16 = 0 (default) keeps the
+ * reference's FP32 V-cycles; fp16 = 1 stores the V-cycle vectors u, b and r of
+ * every level in FP16 with FP32 arithmetic (the smoothing sweep then moves
+ * 10 B per cell instead of 16). The preconditioner is linear, so it runs on
+ * r / max|r| of each problem and scales its result back, keeping the stored
+ * values inside FP16's normal range; the interface solver, the PCG vectors and
+ * reductions stay FP32/FP64. hdd_smooth and hdd_profile_sweep then act on the
+ * FP16 buffers; hdd_vcycle stays FP32. */
+hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16);
+
 /* Releases all device and host memory of a context. */
 hdd_status hdd_destroy(hdd_ctx* ctx);
 
@@ -189,7 +200,8 @@ hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* seconds);
 /* Measurement: runs `reps` level-0 smoothing sweeps (all Active cells) on the
  * context stream bracketed by HIP events; returns the mean duration of one
  * launch and the algorithmic HBM bytes of one launch (16 B per page slot:
- * flags, u and b read, u written). No reference counterpart. */
+ * flags, u and b read, u written; 10 B with FP16 V-cycle storage). No
+ * reference counterpart. */
 hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes_per_launch);
 
 /* Measurement: runs `reps` plain copies of one level-0 float channel into the
