@@ -57,7 +57,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="sphere256", choices=["sphere256", "sphere64"])
+    ap.add_argument("--config", default="sphere256", choices=["sphere256", "sphere64", "sparse512"],
+                    help="sparse512: the 512^3 union-of-spheres domain, FP32 and FP16-storage V-cycles at 1e-6 and 1e-8")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=500)
     ap.add_argument("--batch-count", type=int, default=16, help="problems of the batched leg (0: skip)")
@@ -75,6 +76,9 @@ def main():
 
     from hierarchicaldd_amd import domain, hdd
 
+    if args.config == "sparse512":
+        print(json.dumps(sparse_leg(hdd)), flush=True)
+        return
     t0 = time.time()
     dom, cfg = domain.config1() if args.config == "sphere256" else domain.config0()
     log(f"[bench] domain {dom.meta.get('workload')} pages={dom.n_pages} generated in {time.time() - t0:.1f}s")
@@ -191,6 +195,63 @@ def counter_traffic(pages):
     if sweep_grid < pages * 256:
         return None
     return 1024.0 * (fetch["k_smooth"][sweep_grid] * f_scale + write["k_smooth"][sweep_grid] * w_scale)
+
+
+def true_residual(dom, x):
+    """max|b - A x| / max|b| in float64 with the 7-point operator of the input flags
+    (face f of an Active cell contributes x_c - x_n; Dirichlet neighbours are zero)."""
+    from hierarchicaldd_amd import domain
+    fl = dom.dense(dom.flags)
+    xd = dom.dense(np.asarray(x).reshape(dom.n_pages, 256)).astype(np.float64)
+    bd = dom.dense(dom.b).astype(np.float64)
+    act = (fl & domain.ACTIVE) != 0
+    ax = np.zeros_like(xd)
+    xp = np.pad(xd, 1)
+    X, Y, Z = xd.shape
+    nbrs = [xp[0:X, 1:Y + 1, 1:Z + 1], xp[2:X + 2, 1:Y + 1, 1:Z + 1], xp[1:X + 1, 0:Y, 1:Z + 1],
+            xp[1:X + 1, 2:Y + 2, 1:Z + 1], xp[1:X + 1, 1:Y + 1, 0:Z], xp[1:X + 1, 1:Y + 1, 2:Z + 2]]
+    for f, nb in enumerate(nbrs):
+        on = (fl & np.uint32(domain.FACE_MX << f)) != 0
+        ax += np.where(on, xd - nb, 0.0)
+    r = np.where(act, bd - ax, 0.0)
+    return float(np.abs(r).max() / np.abs(bd).max())
+
+
+def sparse_leg(hdd):
+    """The 512^3 union-of-spheres domain: FP32 and FP16-storage V-cycles at 1e-6 and 1e-8."""
+    from hierarchicaldd_amd import domain
+    t0 = time.time()
+    dom, cfg = domain.sparse_spheres()
+    meta = {k: v for k, v in dom.meta.items() if k != "spheres"}
+    log(f"[sparse] {meta} pages={dom.n_pages} generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    h = hdd.DDSolver(dom, cfg, host_only=True)      # pages, flags, interface operators
+    log(f"[sparse] host setup {time.time() - t0:.1f}s")
+    t1 = time.time()
+    g = hdd.DDBatch(from_solvers=[h])               # factors, plans, device upload
+    h.close()
+    st = g.stats()
+    log(f"[sparse] factors and device setup {time.time() - t1:.1f}s (interface dofs {st['interface_dofs']:.0f}, "
+        f"plans {st['sigma_device_bytes'] / 2**30:.2f} GiB)")
+    st["setup_s"] = time.time() - t0
+    bmax = float(np.abs(dom.b).max())
+    rows = []
+    for fp16 in (False, True):
+        g.set_fp16(fp16)
+        for tol in (1e-6, 1e-8):
+            it, hist, secs = g.solve(tol * bmax, 2000)
+            res = true_residual(dom, g.get(hdd.CH_X))
+            row = {"storage": "fp16" if fp16 else "fp32", "tolerance": tol, "iterations": it, "seconds": secs,
+                   "final_max_r_rel": float(hist[-1] / bmax), "true_residual_rel_fp64": res}
+            log(f"[sparse] {row}")
+            rows.append(row)
+    ms, by = g.profile_sweep(50)
+    g.set_fp16(False)
+    ms32, by32 = g.profile_sweep(50)
+    g.close()
+    return {"metric": METRIC, "config": {"workload": "sparse512_spheres24", **meta, "pages": dom.n_pages, "sub_size": cfg.sub_size,
+                                         "vlevels": cfg.vlevels}, "setup_s": st["setup_s"], "solves": rows,
+            "sweep_fp32": {"us": ms32 * 1e3, "GBs": by32 / (ms32 * 1e-3) / 1e9}, "sweep_fp16": {"us": ms * 1e3, "GBs": by / (ms * 1e-3) / 1e9}}
 
 
 def batched_leg(hdd, count, seed=0):
