@@ -1,12 +1,13 @@
 // HIP kernels for gfx950: matrix-free V-cycle on paged SPGrid levels and the
 // outer PCG vector operations.
 //
-// Launch shape: one 256-thread workgroup (four 64-lane wavefronts) per 4x8x8
-// page; thread t owns cell t of its page (t = x*64 + y*8 + z), so every
-// channel access of a wavefront is a contiguous, coalesced 256-byte segment.
-// Face neighbours inside the page are at t +-64, +-8, +-1; across the page
-// face they come from the page's 6-entry neighbour table (the zero page
-// `np` stands in for unallocated pages).
+// Launch shapes: the smoothing sweep and the residual take one 64-lane
+// wavefront per 4x8x8 page, four cells per lane with 16-byte accesses (see
+// Stencil4); the other page kernels one 256-thread workgroup per page, thread
+// t owning cell t (t = x*64 + y*8 + z), so every channel access of a wavefront
+// is a contiguous, coalesced segment. Face neighbours inside a page are at
+// cell offsets +-64, +-8, +-1; across the page face they come from the page's
+// 6-entry neighbour table (the zero page `np` stands in for unallocated pages).
 //
 // Floating-point order reproduces the reference kernels operation by
 // operation (the library is compiled with -ffp-contract=off):
@@ -58,47 +59,111 @@ __device__ __forceinline__ float ld(const __half* __restrict__ p, size_t i) { re
 __device__ __forceinline__ void st(float* __restrict__ p, size_t i, float v) { p[i] = v; }
 __device__ __forceinline__ void st(__half* __restrict__ p, size_t i, float v) { p[i] = __float2half_rn(v); }
 
-// float residual of one cell, reference operation order. The six neighbour
-// values are loaded unconditionally (the zero page backs missing pages, so
-// every address is valid) and selected by the face bits afterwards: the loads
-// then leave together with the cell's own, instead of waiting for its flags.
-template <class S>
-__device__ __forceinline__ float cell_residual(const int* __restrict__ nbr, int p, int c, uint32_t f,
-                                               const S* __restrict__ u, float b) {
-    const float cv = ld(u, p * 256 + c);
-    float nv[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) nv[k] = ld(u, nb_index(nbr, p, c, k));
-    float result = 0.f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        if (f & (kFaceMX << k)) result -= (nv[k] - cv);
-    return b - result;
+// Stencil sweeps: one 64-lane wavefront per page, lane l owns the four cells
+// 4l .. 4l+3 (z = 4(l&1) .. +3 of row x = l>>4, y = (l>>1)&7) and moves them
+// with one 16-byte access per channel (8 bytes with FP16 storage). Face
+// neighbours inside the page are the vectors of lanes l+-16 (x) and l+-2 (y)
+// and the neighbouring elements (z); across a page face they come from the
+// page's 6-entry neighbour table (the zero page `np` stands in for unallocated
+// pages). Four pages per 256-thread workgroup.
+struct V4 { float v[4]; };
+__device__ __forceinline__ V4 ld4(const float* __restrict__ p, size_t i4) {
+    const float4 t = reinterpret_cast<const float4*>(p)[i4];
+    return {{t.x, t.y, t.z, t.w}};
+}
+__device__ __forceinline__ V4 ld4(const __half* __restrict__ p, size_t i4) {
+    const uint2 t = reinterpret_cast<const uint2*>(p)[i4];
+    __half2 a, b;
+    __builtin_memcpy(&a, &t.x, 4);
+    __builtin_memcpy(&b, &t.y, 4);
+    const float2 fa = __half22float2(a), fb = __half22float2(b);
+    return {{fa.x, fa.y, fb.x, fb.y}};
+}
+__device__ __forceinline__ void st4(float* __restrict__ p, size_t i4, const V4& v) {
+    reinterpret_cast<float4*>(p)[i4] = make_float4(v.v[0], v.v[1], v.v[2], v.v[3]);
+}
+__device__ __forceinline__ void st4(__half* __restrict__ p, size_t i4, const V4& v) {
+    const __half2 a = __floats2half2_rn(v.v[0], v.v[1]), b = __floats2half2_rn(v.v[2], v.v[3]);
+    uint2 t;
+    __builtin_memcpy(&t.x, &a, 4);
+    __builtin_memcpy(&t.y, &b, 4);
+    reinterpret_cast<uint2*>(p)[i4] = t;
 }
 
-// One damped-Jacobi sweep (residual + correction fused, ping-pong buffers).
-// Cells outside the selection are copied unchanged.
+// The four cells of lane l of page p with their six face neighbours, and the
+// float residual of each, in the reference operation order
+// (r = b - sum over set face bits (-x,+x,-y,+y,-z,+z) of -(u_n - u_c),
+// LIB/SPGrid_Residual_Legacy.h:52-61). Neighbour values are loaded whether or
+// not their face bit is set (every address is valid) and selected afterwards.
+template <class S>
+struct Stencil4 {
+    V4 uc, r;
+    uint32_t f[4];
+    __device__ __forceinline__ Stencil4(const DevLevel& L, int p, int l, const S* __restrict__ u, const S* __restrict__ b) {
+        const int x = l >> 4, y = (l >> 1) & 7, zh = l & 1;
+        const int* nb = L.nbr + p * 6;
+        const size_t base = (size_t)p * 64;
+        const uint4 fv = reinterpret_cast<const uint4*>(L.flags)[base + l];
+        f[0] = fv.x; f[1] = fv.y; f[2] = fv.z; f[3] = fv.w;
+        uc = ld4(u, base + l);
+        const V4 xm = ld4(u, x > 0 ? base + l - 16 : (size_t)nb[0] * 64 + l + 48);
+        const V4 xp = ld4(u, x < 3 ? base + l + 16 : (size_t)nb[1] * 64 + l - 48);
+        const V4 ym = ld4(u, y > 0 ? base + l - 2 : (size_t)nb[2] * 64 + l + 14);
+        const V4 yp = ld4(u, y < 7 ? base + l + 2 : (size_t)nb[3] * 64 + l - 14);
+        const float zm = ld(u, zh ? (size_t)p * 256 + 4 * l - 1 : (size_t)nb[4] * 256 + 4 * l + 7);
+        const float zp = ld(u, zh ? (size_t)nb[5] * 256 + 4 * l - 4 : (size_t)p * 256 + 4 * l + 4);
+        const V4 bv = ld4(b, base + l);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float cv = uc.v[k];
+            const float nv[6] = {xm.v[k], xp.v[k], ym.v[k], yp.v[k], k > 0 ? uc.v[k - 1] : zm, k < 3 ? uc.v[k + 1] : zp};
+            float result = 0.f;
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (f[k] & (kFaceMX << q)) result -= (nv[q] - cv);
+            r.v[k] = bv.v[k] - result;
+        }
+    }
+};
+
+__host__ __device__ inline int page_groups(int np) { return (np + 3) >> 2; }
+
+// One damped-Jacobi sweep (residual + correction fused, ping-pong buffers);
+// u += (omega / face count) r on the selected cells
+// (LIB/SPGrid_Correction_Legacy.h:60-77), other cells copied unchanged.
 template <class S>
 __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const S* __restrict__ u_in, S* __restrict__ u_out,
                                                 const S* __restrict__ b, int write_interior, int write_boundary) {
-    const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= L.np || page_stopped(L, p)) return;
-    const uint32_t f = L.flags[e];
-    const uint32_t sel = f & (kActive | kBoundary);
-    const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
-    float u = ld(u_in, e);
-    const float r = cell_residual(L.nbr, p, c, f, u_in, ld(b, e));
-    if (w) u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * r;
-    st(u_out, e, u);
+    const int ng = page_groups(L.np), gi = xcd_page(ng), l = threadIdx.x & 63;
+    const int p = gi * 4 + (int)(threadIdx.x >> 6);
+    if (gi >= ng || p >= L.np || page_stopped(L, p)) return;
+    const Stencil4<S> st(L, p, l, u_in, b);
+    V4 out;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t f = st.f[k], sel = f & (kActive | kBoundary);
+        const bool w = (sel == kActive && write_interior) || (sel == (kActive | kBoundary) && write_boundary);
+        float u = st.uc.v[k];
+        if (w) u += c_omega_fc[__popc(f & (0x3fu * kFaceMX))] * st.r.v[k];
+        out.v[k] = u;
+    }
+    st4(u_out, (size_t)p * 64 + l, out);
 }
 
+// residual at Active cells; other cells of r are written 0 (the value they
+// always hold: only this kernel writes r, and the reference clears it before
+// each V-cycle, DDS/SPGrid_Domain_Decomposition_Preconditioner.h:93,120)
 template <class S>
 __global__ void __launch_bounds__(256) k_residual(DevLevel L, const S* __restrict__ u, const S* __restrict__ b,
                                                   S* __restrict__ r) {
-    const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= L.np || page_stopped(L, p)) return;
-    const uint32_t f = L.flags[e];
-    if (f & kActive) st(r, e, cell_residual(L.nbr, p, c, f, u, ld(b, e)));
+    const int ng = page_groups(L.np), gi = xcd_page(ng), l = threadIdx.x & 63;
+    const int p = gi * 4 + (int)(threadIdx.x >> 6);
+    if (gi >= ng || p >= L.np || page_stopped(L, p)) return;
+    const Stencil4<S> st(L, p, l, u, b);
+    V4 out;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out.v[k] = (st.f[k] & kActive) ? st.r.v[k] : 0.f;
+    st4(r, (size_t)p * 64 + l, out);
 }
 
 template <class S>
@@ -343,11 +408,12 @@ void upload_constants() {
 
 template <class S>
 static void smooth_t(const DevLevel& L, const S* u_in, S* u_out, const S* b, bool interior, bool boundary, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_smooth<S>, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u_in, u_out, b, (int)interior, (int)boundary);
+    if (L.np) hipLaunchKernelGGL(k_smooth<S>, dim3(kXcds * xcd_chunk(page_groups(L.np))), dim3(256), 0, s, L, u_in, u_out, b, (int)interior,
+                                 (int)boundary);
 }
 template <class S>
 static void residual_t(const DevLevel& L, const S* u, const S* b, S* r, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_residual<S>, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, u, b, r);
+    if (L.np) hipLaunchKernelGGL(k_residual<S>, dim3(kXcds * xcd_chunk(page_groups(L.np))), dim3(256), 0, s, L, u, b, r);
 }
 template <class S>
 static void fill_t(S* a, float v, size_t n, hipStream_t s) {
