@@ -94,6 +94,9 @@ def main():
         log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic")
         copy_ms, copy_bytes = solver.profile_copy(20)
         log(f"[bench] calibration copy: {copy_ms * 1e3:.1f} us/launch, {copy_bytes / 2:.0f} bytes read and {copy_bytes / 2:.0f} written")
+        solver.set_fp16(True)
+        h_ms, h_bytes = solver.profile_sweep(20)
+        log(f"[bench] level-0 sweep, FP16 storage: {h_ms * 1e3:.1f} us/launch, {h_bytes / 1e6:.1f} MB algorithmic")
         return
 
     for w in range(args.warmup):
@@ -127,6 +130,7 @@ def main():
     fp16 = {"time_to_tolerance_s": float(np.mean(t16)), "solves_per_s": 1.0 / float(np.mean(t16)), "iterations": int(np.median(it16)),
             "sweep_us": h_ms * 1e3, "sweep_achieved_GBs": h_bytes / (h_ms * 1e-3) / 1e9,
             "sweep_frac": h_bytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "sweep_bytes_per_slot": 10}
+    fp16["traffic"] = counter_traffic(dom.n_pages, "k_smooth<__half>") if args.config == "sphere256" else None
     log(f"[bench] fp16-storage V-cycles: {fp16['iterations']} iterations, {fp16['time_to_tolerance_s']:.3f}s; "
         f"level-0 sweep {h_ms * 1e3:.1f} us ({fp16['sweep_achieved_GBs']:.0f} GB/s algorithmic at 10 B/slot)")
     solver.close()
@@ -165,13 +169,20 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def counter_traffic(pages):
+def sweep_grid_threads(pages):
+    """Launch grid (threads) of the level-0 sweep: four pages per 256-thread workgroup, XCD-chunked."""
+    groups = (pages + 3) // 4
+    return 8 * ((groups + 7) // 8) * 256
+
+
+def counter_traffic(pages, kernel="k_smooth<float>"):
     """HBM bytes of one level-0 sweep from the committed rocprofv3 counter summaries.
 
-    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the sweep kernel, each scaled by
-    what the same counter reads for the calibration copy kernel of the same runs,
-    which reads and writes (pages + 1) * 256 floats per launch (hdd_profile_copy).
-    None when the summaries are absent or were taken on another page count.
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the sweep kernel at its
+    level-0 grid, each scaled by what the same counter reads for the
+    calibration copy kernel of the same runs, which reads and writes
+    (pages + 1) * 256 floats per launch (hdd_profile_copy). None when the
+    summaries are absent or were taken on another page count.
     """
     def read(name, counter):
         path = os.path.join(PROFILES, name)
@@ -181,20 +192,20 @@ def counter_traffic(pages):
         for line in open(path):
             f = line.split()
             if len(f) >= 5 and f[-3] == counter and f[-4].isdigit():
-                vals.setdefault(f[0].split("::")[-1], {})[int(f[-4])] = float(f[-1])
+                vals.setdefault(" ".join(f[:-4]).split("::")[-1], {})[int(f[-4])] = float(f[-1])
         return vals
     fetch = read("k_smooth_sphere256_fetch_size.txt", "FETCH_SIZE")
     write = read("k_smooth_sphere256_write_size.txt", "WRITE_SIZE")
-    if not fetch or not write or "k_smooth" not in fetch or "k_copy" not in fetch:
+    if not fetch or not write or kernel not in fetch or kernel not in write or "k_copy" not in fetch:
         return None
     copy_kib = (pages + 1) * 256 * 4 / 1024.0
     copy_grid = max(fetch["k_copy"], key=lambda g: fetch["k_copy"][g])
     f_scale = copy_kib / fetch["k_copy"][copy_grid]
     w_scale = copy_kib / write["k_copy"][copy_grid]
-    sweep_grid = max(fetch["k_smooth"], key=lambda g: g)
-    if sweep_grid < pages * 256:
+    grid = sweep_grid_threads(pages)
+    if grid not in fetch[kernel] or grid not in write[kernel]:
         return None
-    return 1024.0 * (fetch["k_smooth"][sweep_grid] * f_scale + write["k_smooth"][sweep_grid] * w_scale)
+    return 1024.0 * (fetch[kernel][grid] * f_scale + write[kernel][grid] * w_scale)
 
 
 def true_residual(dom, x):
