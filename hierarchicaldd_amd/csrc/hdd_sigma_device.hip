@@ -214,10 +214,14 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
 }
 
 // Heights whose supernodes all have <= kSliceRows (128) rows: one workgroup per
-// supernode does the whole height's work for it in one launch.
+// run of up to M supernodes that share their blocks (members of identical
+// factors; runs of one otherwise) does the whole height's work for them in one
+// launch. Every block entry is loaded once and applied to each member in turn,
+// and each member's sums run in exactly the order of a one-member run.
 // forward: t = b - (incidence sums) -> y = Linv t (LDS) -> c_g = L21_g y
-template <int G>
-__global__ void __launch_bounds__(256) k_tri_fwd_small(int q0, const int* __restrict__ sn_list, const int* __restrict__ s0,
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                       const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
                                                        const long long* __restrict__ doff, const float* __restrict__ diag,
                                                        const int* __restrict__ inc_ptr, const long long* __restrict__ cpos,
@@ -225,50 +229,80 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int q0, const int* __rest
                                                        const long long* __restrict__ loff, const long long* __restrict__ cofs,
                                                        const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    __shared__ float ts[TriPlanHost::kSliceRows], ys[TriPlanHost::kSliceRows];
-    const int g = sn_list[q0 + blockIdx.x], n = sz[g], base = s0[g], m = nr[g];
-    if (sn_stopped(sn_prob, active, g)) return;
-    for (int k = threadIdx.x; k < n; k += 256) {
-        const int i = base + k;
+    __shared__ float ts[M][TriPlanHost::kSliceRows], ys[M][TriPlanHost::kSliceRows];
+    __shared__ int base_s[M];
+    const int q0 = run_q[r0 + blockIdx.x], cnt = run_n[r0 + blockIdx.x];
+    const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
+    bool any = false;
+    for (int j = 0; j < cnt; ++j) any = any || !sn_stopped(sn_prob, active, sn_list[q0 + j]);
+    if (!any) return;
+    if (threadIdx.x < cnt) base_s[threadIdx.x] = s0[sn_list[q0 + threadIdx.x]];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt * n; idx += 256) {
+        const int j = idx / n, k = idx - j * n;
+        const int i = base_s[j] + k;
         float sum = 0.f;
         for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
-        ts[k] = b[orig_of[i]] - sum;
+        ts[j][k] = b[orig_of[i]] - sum;
     }
     __syncthreads();
-    const float* D = diag + doff[g];
+    const float* D = diag + doff[g0];
     const int lane = threadIdx.x & (G - 1);
     constexpr int kU = TriPlanHost::kSliceRows / G;  // fixed trip count: every load of a row is issued at once
     for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
         const float* Dr = D + (long long)r * n;
-        float acc = 0.f;
+        float acc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] = 0.f;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int k = lane + u * G;
-            if (k <= r) acc += Dr[k] * ts[k];
+            if (k <= r) {
+                const float d = Dr[k];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) acc[j] += d * ts[j][k];
+            }
         }
-        acc = group_sum<G>(acc);
-        if (lane == 0) { ys[r] = acc; y[base + r] = acc; }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < cnt) {
+                const float v = group_sum<G>(acc[j]);
+                if (lane == 0) { ys[j][r] = v; y[base_s[j] + r] = v; }
+            }
     }
     __syncthreads();
-    const float* L = l21 + loff[g];
-    float* cg = c + cofs[g];
+    const float* L = l21 + loff[g0];
     for (int a = (int)threadIdx.x / G; a < m; a += 256 / G) {
         const float* La = L + (long long)a * n;
-        float acc = 0.f;
+        float acc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] = 0.f;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            const int j = lane + u * G;
-            if (j < n) acc += La[j] * ys[j];
+            const int k = lane + u * G;
+            if (k < n) {
+                const float l = La[k];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) acc[j] += l * ys[j][k];
+            }
         }
-        acc = group_sum<G>(acc);
-        if (lane == 0) cg[a] = acc;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < cnt) {
+                const float v = group_sum<G>(acc[j]);
+                if (lane == 0) c[cofs[sn_list[q0 + j]] + a] = v;
+            }
     }
 }
 
 // backward: t_j = y_j - sum_a L21_g[a, j] z[rows_g[a]] (four wavefronts split
-// the rows, lanes own columns j and j + 64, LDS reduction) -> z = Linv^T t
-template <int G>
-__global__ void __launch_bounds__(256) k_tri_bwd_small(int q0, const int* __restrict__ sn_list, const int* __restrict__ s0,
+// the rows: wavefront w takes rows a = w mod 4 in ascending order; lanes own
+// columns j and j + 64; LDS reduction) -> z = Linv^T t
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                       const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
                                                        const int* __restrict__ roff, const int* __restrict__ srows,
                                                        const long long* __restrict__ loff, const float* __restrict__ l21,
@@ -276,49 +310,81 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int q0, const int* __rest
                                                        const int* __restrict__ orig_of, const float* __restrict__ y,
                                                        float* __restrict__ z, float* __restrict__ x,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    constexpr int kChunk = 1024;
-    __shared__ float zr[kChunk];
-    __shared__ float part[4][TriPlanHost::kSliceRows];
-    __shared__ float ts[TriPlanHost::kSliceRows];
-    const int g = sn_list[q0 + blockIdx.x], n = sz[g], base = s0[g], m = nr[g];
-    if (sn_stopped(sn_prob, active, g)) return;
+    constexpr int kChunk = 128;  // a multiple of 4: the rows' order per wavefront does not depend on it
+    __shared__ float zr[M][kChunk];
+    __shared__ float part[4][M][TriPlanHost::kSliceRows];
+    __shared__ float ts[M][TriPlanHost::kSliceRows];
+    __shared__ int base_s[M], roff_s[M];
+    const int q0 = run_q[r0 + blockIdx.x], cnt = run_n[r0 + blockIdx.x];
+    const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
+    bool any = false;
+    for (int j = 0; j < cnt; ++j) any = any || !sn_stopped(sn_prob, active, sn_list[q0 + j]);
+    if (!any) return;
+    if (threadIdx.x < cnt) { base_s[threadIdx.x] = s0[sn_list[q0 + threadIdx.x]]; roff_s[threadIdx.x] = roff[sn_list[q0 + threadIdx.x]]; }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int* R = srows + roff[g];
-    const float* L = l21 + loff[g];
+    const float* L = l21 + loff[g0];
     const int j0 = min(lane, n - 1), j1 = min(lane + 64, n - 1);
-    float acc0 = 0.f, acc1 = 0.f;
+    float acc0[M], acc1[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) { acc0[j] = 0.f; acc1[j] = 0.f; }
+    __syncthreads();
     for (int a0 = 0; a0 < m; a0 += kChunk) {
         const int cm = min(kChunk, m - a0);
-        for (int q = threadIdx.x; q < cm; q += 256) zr[q] = z[R[a0 + q]];
+        for (int idx = threadIdx.x; idx < cnt * cm; idx += 256) {
+            const int j = idx / cm, q = idx - j * cm;
+            zr[j][q] = z[srows[roff_s[j] + a0 + q]];
+        }
         __syncthreads();
 #pragma unroll 4
         for (int q = w; q < cm; q += 4) {
             const float* La = L + (long long)(a0 + q) * n;
-            const float zq = zr[q];
-            acc0 += La[j0] * zq;
-            acc1 += La[j1] * zq;
+            const float l0 = La[j0], l1 = La[j1];
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < cnt) {
+                    const float zq = zr[j][q];
+                    acc0[j] += l0 * zq;
+                    acc1[j] += l1 * zq;
+                }
         }
         __syncthreads();
     }
-    if (lane < n) part[w][lane] = acc0;
-    if (lane + 64 < n) part[w][lane + 64] = acc1;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < cnt) {
+            if (lane < n) part[w][j][lane] = acc0[j];
+            if (lane + 64 < n) part[w][j][lane + 64] = acc1[j];
+        }
     __syncthreads();
-    for (int k = threadIdx.x; k < n; k += 256)
-        ts[k] = y[base + k] - ((part[0][k] + part[1][k]) + (part[2][k] + part[3][k]));
+    for (int idx = threadIdx.x; idx < cnt * n; idx += 256) {
+        const int j = idx / n, k = idx - j * n;
+        ts[j][k] = y[base_s[j] + k] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k]));
+    }
     __syncthreads();
-    const float* DT = diagT + doff[g];
+    const float* DT = diagT + doff[g0];
     const int gl = threadIdx.x & (G - 1);
     constexpr int kU = TriPlanHost::kSliceRows / G;
     for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
         const float* Dr = DT + (long long)r * n;
-        float acc = 0.f;
+        float acc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] = 0.f;
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const int k = r + gl + u * G;
-            if (k < n) acc += Dr[k] * ts[k];
+            if (k < n) {
+                const float d = Dr[k];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) acc[j] += d * ts[j][k];
+            }
         }
-        acc = group_sum<G>(acc);
-        if (gl == 0) { z[base + r] = acc; x[orig_of[base + r]] = acc; }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < cnt) {
+                const float v = group_sum<G>(acc[j]);
+                if (gl == 0) { z[base_s[j] + r] = v; x[orig_of[base_s[j] + r]] = v; }
+            }
     }
 }
 
@@ -463,6 +529,8 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
         sn_prob.upload(sp);
     }
     sn_begin = P.sn_begin; sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
+    run_begin = P.run_begin; run_max = P.run_max;
+    run_q.upload(P.run_q); run_n.upload(P.run_n);
     max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
     orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
     sn_of.upload(P.sn_of); sn_list.upload(P.sn_list); sl_sn.upload(P.sl_sn); sl_r0.upload(P.sl_r0); rowlist.upload(P.rowlist); inc_ptr.upload(P.inc_ptr);
@@ -474,7 +542,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
 }
 
 size_t TriPlan::bytes() const {
-    return (orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_prob.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
+    return (run_q.n + run_n.n + orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_prob.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
             ci_a1.n + bi_sn.n + bi_c.n + bi_j0.n) * 4 +
            (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4;
 }
@@ -487,6 +555,18 @@ size_t TriPlan::bytes() const {
         case 32: hipLaunchKernelGGL(KERNEL<32>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;            \
         default: hipLaunchKernelGGL(KERNEL<64>, dim3(BLOCKS), dim3(256), SHMEM, __VA_ARGS__); break;            \
     }
+
+static constexpr int kRunM = TriPlanHost::kRunMembers;
+#define HDD_RUN_DISPATCH(G, M, KERNEL, BLOCKS, STREAM, ...)                                                       \
+    do {                                                                                                        \
+        switch (G) {                                                                                            \
+            case 4: hipLaunchKernelGGL((KERNEL<4, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            case 8: hipLaunchKernelGGL((KERNEL<8, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            case 16: hipLaunchKernelGGL((KERNEL<16, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            case 32: hipLaunchKernelGGL((KERNEL<32, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<64, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+        }                                                                                                       \
+    } while (0)
 
 #define HDD_ROWS_DISPATCH(G, KERNEL, ROWS, ...)                                                                \
     switch (G) {                                                                                                \
@@ -504,11 +584,19 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     const int* SP = sn_prob.p;
     for (int h = 0; h <= H; ++h) {
         if (fused[h]) {
-            const int q0 = sn_begin[h], nq = sn_begin[h + 1] - sn_begin[h];
-            if (nq)
-                HDD_SN_DISPATCH(G[h], k_tri_fwd_small, nq, 0, s, q0, (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p,
-                                (const long long*)doff.p, (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p,
-                                (const int*)orig_of.p, b, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
+            const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
+            if (nr_) {
+                if (run_max[h] > 1)
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
+                                     (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
+                                     (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
+                else
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
+                                     (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
+                                     (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
+            }
             continue;
         }
         const int nrows = row_begin[h + 1] - row_begin[h];
@@ -530,11 +618,19 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     }
     for (int h = H; h >= 0; --h) {
         if (fused[h]) {
-            const int q0 = sn_begin[h], nq = sn_begin[h + 1] - sn_begin[h];
-            if (nq)
-                HDD_SN_DISPATCH(G[h], k_tri_bwd_small, nq, 0, s, q0, (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p,
-                                (const int*)roff.p, (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
-                                (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
+            const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
+            if (nr_) {
+                if (run_max[h] > 1)
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
+                                     (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
+                                     (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
+                else
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
+                                     (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
+                                     (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
+            }
             continue;
         }
         const int i0 = bi_begin[h], ni = bi_begin[h + 1] - bi_begin[h];

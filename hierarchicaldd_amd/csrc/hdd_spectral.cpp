@@ -349,19 +349,20 @@ void ParallelSolve8::solve(const double* B, double* X) const {
     if (n_top == 0) {
         solve_interleaved<P>(F, y.data());
     } else {
-        const int threads = std::max(1, omp_get_max_threads());
-        std::vector<double> bufs((size_t)threads * n_top * P, 0.0);
+        // one buffer per subtree, summed in subtree order: the result does not
+        // depend on which thread ran which subtree
+        std::vector<double> bufs((size_t)nt * n_top * P, 0.0);
 #pragma omp parallel for schedule(dynamic, 1)
         for (int t = 0; t < nt; ++t) {
-            double* buf = &bufs[(size_t)omp_get_thread_num() * n_top * P];
+            double* buf = &bufs[(size_t)t * n_top * P];
             for (int s = task_first[t]; s <= task_last[t]; ++s) forward_sn<P>(F, s, y.data(), split[s], buf, top_slot.data());
         }
 #pragma omp parallel for schedule(static)
         for (int c = 0; c < n; ++c) {
             const int slot = top_slot[c];
             if (slot < 0) continue;
-            for (int th = 0; th < threads; ++th)
-                for (int a = 0; a < P; ++a) y[(size_t)c * P + a] += bufs[((size_t)th * n_top + slot) * P + a];
+            for (int t = 0; t < nt; ++t)
+                for (int a = 0; a < P; ++a) y[(size_t)c * P + a] += bufs[((size_t)t * n_top + slot) * P + a];
         }
         for (int s : top) forward_sn_wide<P>(F, s, y.data());
         for (auto it = top.rbegin(); it != top.rend(); ++it) backward_sn_wide<P>(F, *it, y.data());

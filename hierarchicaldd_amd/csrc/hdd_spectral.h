@@ -30,8 +30,9 @@ void factor_solve_block(const SupernodalFactor& F, int p, const double* B, doubl
 
 // Multi-threaded solve with a supernodal factor, 8 right-hand sides at a time.
 // Disjoint subtrees of the elimination tree run in parallel; their updates of
-// rows that belong to the shared ancestors go to per-thread buffers that are
-// summed before the shared top of the tree is solved on one thread. The
+// rows that belong to the shared ancestors go to per-subtree buffers that are
+// summed in subtree order (so the result does not depend on the thread
+// schedule) before the shared top of the tree is solved on one thread. The
 // backward pass runs the top first, then the subtrees in parallel.
 class ParallelSolve8 {
 public:

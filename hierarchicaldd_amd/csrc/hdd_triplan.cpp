@@ -143,6 +143,24 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         if (contrib_rows > 0) P.Gc[h] = lanes_for(contrib_len / contrib_rows);
     }
     P.sn_begin[nh] = (int)P.sn_list.size();
+    // runs of block-sharing supernodes (sn_list is ordered by block offsets within each height)
+    P.run_begin.assign(nh + 1, 0);
+    P.run_max.assign(nh, 1);
+    for (int h = 0; h < nh; ++h) {
+        P.run_begin[h] = (int)P.run_q.size();
+        for (int q = P.sn_begin[h]; q < P.sn_begin[h + 1];) {
+            const int g = P.sn_list[q];
+            int k = 1;
+            while (k < TriPlanHost::kRunMembers && q + k < P.sn_begin[h + 1] && P.doff[P.sn_list[q + k]] == P.doff[g] &&
+                   P.loff[P.sn_list[q + k]] == P.loff[g])
+                ++k;
+            P.run_q.push_back(q);
+            P.run_n.push_back(k);
+            P.run_max[h] = std::max(P.run_max[h], k);
+            q += k;
+        }
+    }
+    P.run_begin[nh] = (int)P.run_q.size();
     P.sl_begin[nh] = (int)P.sl_sn.size();
     P.row_begin[nh] = (int)P.rowlist.size();
     P.ci_begin[nh] = (int)P.ci_sn.size();

@@ -45,6 +45,12 @@ struct TriPlanHost {
     static constexpr int kFusedMinSupernodes = 1024;  // a height is done by one workgroup per supernode only with this many
     std::vector<int> sn_begin, max_sz, big;   // per height: supernode range in sn_list, largest supernode, 1 if > kSliceRows
     std::vector<int> fused;                   // per height: 1 if every supernode is <= kSliceRows and there are many
+    // Runs: consecutive supernodes of sn_list at one height that share their
+    // blocks (members of identical factors), at most kRunMembers each; a fused
+    // height is done by one workgroup per run, which reads the shared blocks once.
+    static constexpr int kRunMembers = 8;
+    std::vector<int> run_q, run_n, run_begin; // run start in sn_list, members; runs by height
+    std::vector<int> run_max;                 // per height: largest run
     std::vector<int> sn_list;
     std::vector<int> sl_sn, sl_r0, sl_begin;  // row slices (supernode, first row) by height
     std::vector<int> sl_rows;                 // rows per slice, per height
