@@ -34,7 +34,13 @@ struct TriPlan {
     // per-problem flags (device, 1 = iterating): supernodes of stopped problems are skipped; nullptr: none skipped
     const int* active = nullptr;
     void solve(const float* b_orig, float* x_orig, hipStream_t s);
+    void fwd_height(int h, const float* b_orig, hipStream_t s);
+    void bwd_height(int h, float* x_orig, hipStream_t s);
+    // launch-shape autotuner (slice rows, contribution items per height); returns the choices changed
+    int tune(hipStream_t s);
     size_t bytes() const;
+    std::vector<int> ci_entries, h_sn_list, h_sz, h_nr;  // host copies the tuner rebuilds the lists from
+    void rebuild_lists();
 };
 
 struct SigmaLevel {
@@ -86,6 +92,7 @@ struct SigmaDevice {
     int bottom_steps = 0;          // block Krylov steps of the eigen-solves (largest)
     double bottom_seconds = 0;     // host time of the eigen-solves
     double bottom_check = -1;      // max |direct+correction - 200 sweeps| / max |200 sweeps| on a random rhs
+    int tuned_choices = 0;         // launch shapes the autotuner changed from the plans' defaults
     TriPlan Lbottom;
     DBuf<float> bot_rhs, bot_sol, spec_V, spec_SV, spec_coef;
     DBuf<double> spec_w1, spec_w2, spec_part;

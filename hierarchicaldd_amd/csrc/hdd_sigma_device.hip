@@ -529,6 +529,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
         sn_prob.upload(sp);
     }
     sn_begin = P.sn_begin; sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
+    ci_entries = P.ci_entries; h_sn_list = P.sn_list; h_sz = P.sz; h_nr = P.nr;
     run_begin = P.run_begin; run_max = P.run_max;
     run_q.upload(P.run_q); run_n.upload(P.run_n);
     max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
@@ -577,12 +578,10 @@ static constexpr int kRunM = TriPlanHost::kRunMembers;
         default: hipLaunchKernelGGL(KERNEL<64>, dim3(blocks_for((ROWS) * 64, 256)), dim3(256), 0, __VA_ARGS__); break; \
     }
 
-void TriPlan::solve(const float* b, float* x, hipStream_t s) {
-    if (n == 0) return;
+void TriPlan::fwd_height(int h, const float* b, hipStream_t s) {
     float* yf = y.p;
-    float* zb = y.p + n;
     const int* SP = sn_prob.p;
-    for (int h = 0; h <= H; ++h) {
+    {
         if (fused[h]) {
             const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
             if (nr_) {
@@ -597,7 +596,7 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
                                      (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
                                      (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
             }
-            continue;
+            return;
         }
         const int nrows = row_begin[h + 1] - row_begin[h];
         const int* rows = rowlist.p + row_begin[h];
@@ -616,7 +615,13 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
                             (const int*)s0.p, (const int*)sz.p, (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p,
                             (const float*)yf, c.p, SP, active);
     }
-    for (int h = H; h >= 0; --h) {
+}
+
+void TriPlan::bwd_height(int h, float* x, hipStream_t s) {
+    float* yf = y.p;
+    float* zb = y.p + n;
+    const int* SP = sn_prob.p;
+    {
         if (fused[h]) {
             const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
             if (nr_) {
@@ -631,7 +636,7 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
                                      (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
                                      (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
             }
-            continue;
+            return;
         }
         const int i0 = bi_begin[h], ni = bi_begin[h + 1] - bi_begin[h];
         if (ni)
@@ -650,6 +655,71 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
                             big[h] ? (const float*)t.p : (const float*)nullptr, (const long long*)bofs.p, (const float*)bpart.p,
                             (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
     }
+}
+
+void TriPlan::solve(const float* b, float* x, hipStream_t s) {
+    if (n == 0) return;
+    for (int h = 0; h <= H; ++h) fwd_height(h, b, s);
+    for (int h = H; h >= 0; --h) bwd_height(h, x, s);
+}
+
+void TriPlan::rebuild_lists() {
+    std::vector<int> a, b_, c_, d, e;
+    build_slices_items(h_sn_list, sn_begin, h_sz, h_nr, sl_rows, ci_entries, a, b_, sl_begin, c_, d, e, ci_begin);
+    sl_sn.upload(a); sl_r0.upload(b_); ci_sn.upload(c_); ci_a0.upload(d); ci_a1.upload(e);
+}
+
+// Launch-shape autotuner: for every height that is not done by per-run
+// workgroups, the rows per diagonal-step slice and the L21 entries per
+// contribution item are chosen by timing the height's forward and backward
+// launches on the device (HIP events, the plan's own scratch as input). Neither
+// choice changes the order of any sum, so the solve's results do not depend on
+// what the timings pick.
+int TriPlan::tune(hipStream_t s) {
+    if (n == 0) return 0;
+    DBuf<float> bx, xx;
+    bx.alloc(n); xx.alloc(n);
+    hipEvent_t e0, e1;
+    hdd_check(hipEventCreate(&e0)); hdd_check(hipEventCreate(&e1));
+    auto time_height = [&](int h) {
+        constexpr int kReps = 8;
+        fwd_height(h, bx.p, s); bwd_height(h, xx.p, s);
+        hdd_check(hipEventRecord(e0, s));
+        for (int r = 0; r < kReps; ++r) { fwd_height(h, bx.p, s); bwd_height(h, xx.p, s); }
+        hdd_check(hipEventRecord(e1, s));
+        hdd_check(hipEventSynchronize(e1));
+        float ms = 0.f;
+        hdd_check(hipEventElapsedTime(&ms, e0, e1));
+        return ms / kReps;
+    };
+    int changed = 0;
+    for (int h = 0; h <= H; ++h) {
+        if (fused[h] || sl_begin[h + 1] == sl_begin[h]) continue;
+        const int sl0 = sl_rows[h], ci0 = ci_entries[h];
+        float best = time_height(h);
+        for (int cand : {8, 16, 32, 64, 128}) {
+            if (cand == sl_rows[h] || (cand > 8 && cand / 2 >= max_sz[h])) continue;
+            const int keep = sl_rows[h];
+            sl_rows[h] = cand;
+            rebuild_lists();
+            const float t = time_height(h);
+            if (t < 0.97f * best) best = t;
+            else { sl_rows[h] = keep; rebuild_lists(); }
+        }
+        for (int cand : {1024, 2048, 4096, 8192, 16384}) {
+            if (cand == ci_entries[h] || ci_begin[h + 1] == ci_begin[h]) continue;
+            const int keep = ci_entries[h];
+            ci_entries[h] = cand;
+            rebuild_lists();
+            const float t = time_height(h);
+            if (t < 0.97f * best) best = t;
+            else { ci_entries[h] = keep; rebuild_lists(); }
+        }
+        changed += (sl_rows[h] != sl0) + (ci_entries[h] != ci0);
+    }
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    hdd_check(hipStreamSynchronize(s));
+    return changed;
 }
 
 SigmaDevice::~SigmaDevice() {}
@@ -689,6 +759,12 @@ static std::vector<int> identical_matrices(const std::vector<Csr>& A) {
 // 200 sweeps leave l^200 of an eigenmode with eigenvalue l; modes with l^200 < 1e-9 are dropped
 static const double kBottomLambdaCut = std::pow(1e-9, 1.0 / 200.0);
 static const size_t kBottomBasisBytes = size_t(2) << 30;
+
+// HDD_AUTOTUNE=0 keeps the plans' default launch shapes
+static bool autotune() {
+    const char* v = std::getenv("HDD_AUTOTUNE");
+    return !(v && std::string(v) == "0");
+}
 
 static std::string bottom_mode() {
     const char* mode = std::getenv("HDD_SIGMA_BOTTOM");
@@ -825,6 +901,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         if (l + 1 < NL) { pr.cols = ifoff[l + 1][NP]; L.P.upload(pr); L.PT.upload(transpose(pr)); }
         L.Lii.build(fi, oi, pi);
         L.Lrr.build(fr, orr, prr);
+        if (autotune()) S->tuned_choices += L.Lii.tune(stream) + L.Lrr.tune(stream);
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
         S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
@@ -870,6 +947,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
             boff += F.bottom_n_int + np;
         }
         S->Lbottom.build(fb, ob, pb);
+        if (autotune()) S->tuned_choices += S->Lbottom.tune(stream);
         S->bot_rhs.alloc(boff); S->bot_sol.alloc(boff);
         S->bot_pos.upload(pos);
         S->bottom_k = m0[NP];

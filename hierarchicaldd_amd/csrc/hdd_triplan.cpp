@@ -99,8 +99,9 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     }
     // per height: supernodes, slices, rows, items
     const int nh = P.H + 1;
-    P.sn_begin.assign(nh + 1, 0); P.sl_begin.assign(nh + 1, 0); P.row_begin.assign(nh + 1, 0);
-    P.ci_begin.assign(nh + 1, 0); P.bi_begin.assign(nh + 1, 0);
+    P.sn_begin.assign(nh + 1, 0); P.row_begin.assign(nh + 1, 0);
+    P.bi_begin.assign(nh + 1, 0);
+    P.ci_entries.assign(nh, TriPlanHost::kContribEntries);
     P.max_sz.assign(nh, 0); P.big.assign(nh, 0); P.fused.assign(nh, 0); P.G.assign(nh, 4); P.Gc.assign(nh, 4); P.Gf.assign(nh, 4); P.Gb.assign(nh, 4);
     P.sl_rows.assign(nh, TriPlanHost::kSliceRows);
     for (int h = 0; h < nh; ++h) {
@@ -110,9 +111,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     }
     for (int h = 0; h < nh; ++h) {
         P.sn_begin[h] = (int)P.sn_list.size();
-        P.sl_begin[h] = (int)P.sl_sn.size();
         P.row_begin[h] = (int)P.rowlist.size();
-        P.ci_begin[h] = (int)P.ci_sn.size();
         P.bi_begin[h] = (int)P.bi_sn.size();
         double rows = 0, diag_len = 0, contrib_rows = 0, contrib_len = 0, incs = 0, chunks = 0;
         std::vector<int> at_h;
@@ -123,14 +122,11 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
             const int n = P.sz[g], nr = P.nr[g];
             P.sn_list.push_back(g);
             P.max_sz[h] = std::max(P.max_sz[h], n);
-            for (int r0 = 0; r0 < n; r0 += P.sl_rows[h]) { P.sl_sn.push_back(g); P.sl_r0.push_back(r0); }
             for (int k = 0; k < n; ++k) {
                 P.rowlist.push_back(P.s0[g] + k);
                 incs += P.inc_ptr[P.s0[g] + k + 1] - P.inc_ptr[P.s0[g] + k];
             }
             rows += n; diag_len += 0.5 * n * (n + 1.0);
-            const int crows = std::max(1, TriPlanHost::kContribEntries / std::max(1, n));
-            for (int a0 = 0; a0 < nr; a0 += crows) { P.ci_sn.push_back(g); P.ci_a0.push_back(a0); P.ci_a1.push_back(std::min(nr, a0 + crows)); }
             contrib_rows += nr; contrib_len += (double)nr * n;
             const int nch = (nr + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
             for (int c = 0; c < nch; ++c)
@@ -161,10 +157,32 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         }
     }
     P.run_begin[nh] = (int)P.run_q.size();
-    P.sl_begin[nh] = (int)P.sl_sn.size();
     P.row_begin[nh] = (int)P.rowlist.size();
-    P.ci_begin[nh] = (int)P.ci_sn.size();
     P.bi_begin[nh] = (int)P.bi_sn.size();
+    build_slices_items(P.sn_list, P.sn_begin, P.sz, P.nr, P.sl_rows, P.ci_entries, P.sl_sn, P.sl_r0, P.sl_begin, P.ci_sn, P.ci_a0,
+                       P.ci_a1, P.ci_begin);
+}
+
+void build_slices_items(const std::vector<int>& sn_list, const std::vector<int>& sn_begin, const std::vector<int>& sz,
+                        const std::vector<int>& nr, const std::vector<int>& sl_rows, const std::vector<int>& ci_entries,
+                        std::vector<int>& sl_sn, std::vector<int>& sl_r0, std::vector<int>& sl_begin, std::vector<int>& ci_sn,
+                        std::vector<int>& ci_a0, std::vector<int>& ci_a1, std::vector<int>& ci_begin) {
+    const int nh = (int)sn_begin.size() - 1;
+    sl_sn.clear(); sl_r0.clear(); ci_sn.clear(); ci_a0.clear(); ci_a1.clear();
+    sl_begin.assign(nh + 1, 0);
+    ci_begin.assign(nh + 1, 0);
+    for (int h = 0; h < nh; ++h) {
+        sl_begin[h] = (int)sl_sn.size();
+        ci_begin[h] = (int)ci_sn.size();
+        for (int q = sn_begin[h]; q < sn_begin[h + 1]; ++q) {
+            const int g = sn_list[q], n = sz[g], m = nr[g];
+            for (int r0 = 0; r0 < n; r0 += sl_rows[h]) { sl_sn.push_back(g); sl_r0.push_back(r0); }
+            const int crows = std::max(1, ci_entries[h] / std::max(1, n));
+            for (int a0 = 0; a0 < m; a0 += crows) { ci_sn.push_back(g); ci_a0.push_back(a0); ci_a1.push_back(std::min(m, a0 + crows)); }
+        }
+    }
+    sl_begin[nh] = (int)sl_sn.size();
+    ci_begin[nh] = (int)ci_sn.size();
 }
 
 void triplan_solve_host(const TriPlanHost& P, const float* b, float* x) {

@@ -53,7 +53,8 @@ struct TriPlanHost {
     std::vector<int> run_max;                 // per height: largest run
     std::vector<int> sn_list;
     std::vector<int> sl_sn, sl_r0, sl_begin;  // row slices (supernode, first row) by height
-    std::vector<int> sl_rows;                 // rows per slice, per height
+    std::vector<int> sl_rows;                 // rows per slice, per height (launch shape: does not change any sum)
+    std::vector<int> ci_entries;              // L21 entries per contribution item, per height (launch shape)
     std::vector<int> rowlist, row_begin;      // plan rows by height (for the reduce kernels)
     std::vector<int> G;                       // lanes per row of the diag kernels, per height
     // forward
@@ -72,6 +73,14 @@ struct TriPlanHost {
 };
 
 void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P);
+
+// (Re)builds the row slices of the diagonal steps and the items of the forward
+// contributions from sn_list, sl_rows and ci_entries. Every row's sum is the
+// same whatever the slice and item sizes, so they can be tuned freely.
+void build_slices_items(const std::vector<int>& sn_list, const std::vector<int>& sn_begin, const std::vector<int>& sz,
+                        const std::vector<int>& nr, const std::vector<int>& sl_rows, const std::vector<int>& ci_entries,
+                        std::vector<int>& sl_sn, std::vector<int>& sl_r0, std::vector<int>& sl_begin, std::vector<int>& ci_sn,
+                        std::vector<int>& ci_a0, std::vector<int>& ci_a1, std::vector<int>& ci_begin);
 
 // Host execution of the plan (double accumulation over its float data), for
 // tests of the plan construction against the factor's own solve.

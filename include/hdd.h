@@ -218,8 +218,11 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * evaluated as one exact solve plus a low-rank correction, else 0, out[9]
  * eigenmodes in that correction, out[10] seconds of its eigen-solve, out[11]
  * block Krylov steps of it (0: dense), out[12] relative max difference of that
- * evaluation against the 200 smooths on a random right-hand side (-1: not run).
- * n = number of slots provided (up to 13). No reference counterpart. */
+ * evaluation against the 200 smooths on a random right-hand side (-1: not run),
+ * out[13] launch shapes of the triangular-solve plans that the setup's
+ * autotuner changed from their defaults (HDD_AUTOTUNE=0 in the environment
+ * turns it off; results do not depend on it).
+ * n = number of slots provided (up to 14). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
 /* Measurement: runs `reps` exact solves of one triangular-solve plan of sigma
