@@ -327,7 +327,7 @@ def served_leg(hdd, n_req, rates, nmax, seed=1):
            "plain_p50_s": float(np.percentile(plain_lat, 50)), "rates": []}
     for lam in rates:
         arr = np.cumsum(rng.exponential(1.0 / lam, size=n_req))
-        srv = hdd.Server(setup_threads=2, groups=2, max_batch=16)
+        srv = hdd.Server(setup_threads=4, groups=2, max_batch=16)
         t0 = time.perf_counter()
         ids = []
         for (dom, cfg), tol, ta in zip(reqs, tols, arr):

@@ -40,7 +40,8 @@ struct TriPlan {
     int tune(hipStream_t s);
     size_t bytes() const;
     std::vector<int> ci_entries, h_sn_list, h_sz, h_nr;  // host copies the tuner rebuilds the lists from
-    void rebuild_lists();
+    static constexpr int kTuneMinRows = 20000;           // plans below this size keep their default shapes
+    void rebuild_lists(hipStream_t s);
 };
 
 struct SigmaLevel {

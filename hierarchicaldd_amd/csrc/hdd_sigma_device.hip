@@ -663,10 +663,17 @@ void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     for (int h = H; h >= 0; --h) bwd_height(h, x, s);
 }
 
-void TriPlan::rebuild_lists() {
+// Rebuilds the slice and item lists into the tuner's fixed-capacity device arrays.
+void TriPlan::rebuild_lists(hipStream_t s) {
     std::vector<int> a, b_, c_, d, e;
     build_slices_items(h_sn_list, sn_begin, h_sz, h_nr, sl_rows, ci_entries, a, b_, sl_begin, c_, d, e, ci_begin);
-    sl_sn.upload(a); sl_r0.upload(b_); ci_sn.upload(c_); ci_a0.upload(d); ci_a1.upload(e);
+    if (a.size() > sl_sn.n || c_.size() > ci_sn.n) throw std::runtime_error("triangular-solve tuner: list capacity exceeded");
+    hdd_check(hipMemcpyAsync(sl_sn.p, a.data(), a.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hdd_check(hipMemcpyAsync(sl_r0.p, b_.data(), b_.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hdd_check(hipMemcpyAsync(ci_sn.p, c_.data(), c_.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hdd_check(hipMemcpyAsync(ci_a0.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hdd_check(hipMemcpyAsync(ci_a1.p, e.data(), e.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    hdd_check(hipStreamSynchronize(s));  // the host vectors go out of scope
 }
 
 // Launch-shape autotuner: for every height that is not done by per-run
@@ -676,7 +683,18 @@ void TriPlan::rebuild_lists() {
 // choice changes the order of any sum, so the solve's results do not depend on
 // what the timings pick.
 int TriPlan::tune(hipStream_t s) {
-    if (n == 0) return 0;
+    if (n < kTuneMinRows) return 0;  // small plans: every launch is latency-bound, little to choose
+    // capacity for the largest lists any candidate produces (8-row slices, 1024-entry items)
+    {
+        const std::vector<int> keep_sl = sl_rows, keep_ci = ci_entries;
+        for (int h = 0; h <= H; ++h) { sl_rows[h] = 8; ci_entries[h] = 1024; }
+        std::vector<int> a, b_, c_, d, e, sb, cb;
+        build_slices_items(h_sn_list, sn_begin, h_sz, h_nr, sl_rows, ci_entries, a, b_, sb, c_, d, e, cb);
+        sl_rows = keep_sl; ci_entries = keep_ci;
+        sl_sn.alloc(a.size()); sl_r0.alloc(a.size()); ci_sn.alloc(std::max<size_t>(1, c_.size()));
+        ci_a0.alloc(std::max<size_t>(1, c_.size())); ci_a1.alloc(std::max<size_t>(1, c_.size()));
+        rebuild_lists(s);
+    }
     DBuf<float> bx, xx;
     bx.alloc(n); xx.alloc(n);
     hipEvent_t e0, e1;
@@ -701,19 +719,19 @@ int TriPlan::tune(hipStream_t s) {
             if (cand == sl_rows[h] || (cand > 8 && cand / 2 >= max_sz[h])) continue;
             const int keep = sl_rows[h];
             sl_rows[h] = cand;
-            rebuild_lists();
+            rebuild_lists(s);
             const float t = time_height(h);
             if (t < 0.97f * best) best = t;
-            else { sl_rows[h] = keep; rebuild_lists(); }
+            else { sl_rows[h] = keep; rebuild_lists(s); }
         }
         for (int cand : {1024, 2048, 4096, 8192, 16384}) {
             if (cand == ci_entries[h] || ci_begin[h + 1] == ci_begin[h]) continue;
             const int keep = ci_entries[h];
             ci_entries[h] = cand;
-            rebuild_lists();
+            rebuild_lists(s);
             const float t = time_height(h);
             if (t < 0.97f * best) best = t;
-            else { ci_entries[h] = keep; rebuild_lists(); }
+            else { ci_entries[h] = keep; rebuild_lists(s); }
         }
         changed += (sl_rows[h] != sl0) + (ci_entries[h] != ci0);
     }
