@@ -1,8 +1,10 @@
 """End-to-end DD-PCG on the GPU against the oracle's CPU dd_pcg.
 
-Bar (north star): iteration count within +-1 of the oracle at the same
-stopping rule (max|r| <= 1e-6 max|b|), residual history close to the
-oracle's, and the final GPU solution's true residual below tolerance.
+Bar: iteration count within +-1 of the oracle at the same stopping rule
+(max|r| <= 1e-6 max|b|), residual history (max|r| per iteration) within 1e-4
+relative of the oracle's over the common prefix (measured 1.4e-6 to 2.2e-5:
+the GPU's interface factors are stored in float, the oracle's solves run in
+double), and the final max|r| below tolerance.
 """
 import numpy as np
 import pytest
@@ -53,4 +55,4 @@ def test_pcg_matches_oracle(pair):
     assert abs(it_g - it_o) <= 1
     assert h_g[-1] <= tol
     # history agreement over the common prefix
-    assert rel.max() < 1e-2
+    assert rel.max() < 1e-4
