@@ -44,7 +44,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", "--offload-arch=gfx950", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", "--offload-arch=gfx950", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])

@@ -2,6 +2,7 @@
 #include "hdd_sigma_device.h"
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 
 namespace hdd {
@@ -81,12 +82,23 @@ size_t ProblemHost::device_bytes() const {
             const SigmaFactors::Level& F = factors->lv[l];
             for (const SupernodalFactor& f : F.Aii) bytes += 8.0 * f.nnz();
             bytes += 8.0 * F.Arr.nnz();
+            // dense tops (m x m floats each); a batch gives them to fewer rows, so this bounds them
+            const int min_rows = dense_top_min_rows();
+            int members = 0, m = 0;
+            for (int s = 0; s < (int)F.rep.size(); ++s) members += min_rows > 0 && F.Aii[F.rep[s]].n >= min_rows;
+            for (const SupernodalFactor& f : F.Aii)
+                if (dense_top_cut(f, min_rows, members, &m) != INT_MAX) bytes += 4.0 * m * m;
+            if (dense_top_cut(F.Arr, min_rows, 1, &m) != INT_MAX) bytes += 4.0 * m * m;
         } else {
             for (const Csr& A : sigma.Aii[l]) bytes += 64.0 * A.nnz();
             bytes += 64.0 * sigma.Arr[l].nnz();
         }
     }
-    if (factors) bytes += 8.0 * factors->bottom.nnz() + 8.0 * factors->spec.V.size();
+    if (factors) {
+        bytes += 8.0 * factors->bottom.nnz() + 8.0 * factors->spec.V.size();
+        int m = 0;
+        if (dense_top_cut(factors->bottom, dense_top_min_rows(), 1, &m) != INT_MAX) bytes += 4.0 * m * m;
+    }
     return (size_t)(bytes * 1.1) + (size_t(4) << 20);
 }
 

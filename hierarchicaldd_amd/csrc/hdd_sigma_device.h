@@ -42,6 +42,13 @@ struct TriPlan {
     std::vector<int> ci_entries, h_sn_list, h_sz, h_nr;  // host copies the tuner rebuilds the lists from
     static constexpr int kTuneMinRows = 20000;           // plans below this size keep their default shapes
     void rebuild_lists(hipStream_t s);
+    // dense tops (hdd_triplan.h): padded top rows, their tops, the Linv blocks
+    int top_nq = 0, Gt = 4;
+    long long top_dense_rows = 0;                        // rows of the distinct dense blocks
+    DBuf<int> top_rows, top_of, top_inc_ptr, top_base, top_ld;
+    DBuf<long long> top_cpos, top_woff;
+    DBuf<float> top_W, top_t;
+    void top_solve(const float* b_orig, float* x_orig, hipStream_t s);
 };
 
 struct SigmaLevel {
@@ -94,6 +101,7 @@ struct SigmaDevice {
     double bottom_seconds = 0;     // host time of the eigen-solves
     double bottom_check = -1;      // max |direct+correction - 200 sweeps| / max |200 sweeps| on a random rhs
     int tuned_choices = 0;         // launch shapes the autotuner changed from the plans' defaults
+    long long dense_top_rows = 0;  // rows of the plans' dense top blocks (all plans)
     TriPlan Lbottom;
     DBuf<float> bot_rhs, bot_sol, spec_V, spec_SV, spec_coef;
     DBuf<double> spec_w1, spec_w2, spec_part;

@@ -17,8 +17,10 @@
 // transposed products in source-row order), built as gathers so no atomics
 // are needed and results are deterministic.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -482,6 +484,85 @@ __global__ void k_spec_apply(int n, const int* __restrict__ row_prob, const int*
     u[i] = acc;
 }
 
+// Dense top of a tree, right-hand side: t_q = b_i - (contributions of the
+// supernodes outside the top to row i = rows[q]), G lanes per row. Padding
+// rows (-1) keep t = 0 from the set-up.
+template <int G>
+__global__ void __launch_bounds__(256) k_top_rhs(int nq, const int* __restrict__ rows, const int* __restrict__ inc_ptr,
+                                                 const long long* __restrict__ cpos, const float* __restrict__ c,
+                                                 const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ t,
+                                                 const int* __restrict__ sn_of, const int* __restrict__ sn_prob,
+                                                 const int* __restrict__ active) {
+    const int q = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
+    if (q >= nq) return;
+    const int i = rows[q];
+    if (i < 0 || sn_stopped(sn_prob, active, sn_of[i])) return;
+    float s = 0.f;
+    for (int p = inc_ptr[q] + lane; p < inc_ptr[q + 1]; p += G) s += c[cpos[p]];
+    s = group_sum<G>(s);
+    if (lane == 0) t[q] = b[orig_of[i]] - s;
+}
+
+// Dense top, both passes over one ld x ld row-major block P holding
+// Linv = L_SS^{-1} twice: P[i][k] = Linv(i,k) for k <= i (rows of Linv) and
+// P[j][i] = Linv(i,j) for i > j (columns of Linv), padding zero. A wavefront
+// per row, float4 loads, fixed-order sums.
+//   forward:  y_i = sum_{k <= i} P[i][k] t_k       (row i, columns [0, i])
+//   backward: z_j = sum_{i >= j} P[j][i] y_i       (row j, columns [j, ld))
+// z goes to the backward pass's input (zb) and to x.
+__device__ __forceinline__ float dot4_masked(const float4 w, const float4 v, int c0, int lo, int hi) {
+    // sum of w[c] v[c] over the columns c0 + e in [lo, hi]
+    float a = 0.f;
+    if (c0 >= lo && c0 <= hi) a += w.x * v.x;
+    if (c0 + 1 >= lo && c0 + 1 <= hi) a += w.y * v.y;
+    if (c0 + 2 >= lo && c0 + 2 <= hi) a += w.z * v.z;
+    if (c0 + 3 >= lo && c0 + 3 <= hi) a += w.w * v.w;
+    return a;
+}
+
+template <bool kForward>
+__global__ void __launch_bounds__(256) k_top_tri(int nq, const int* __restrict__ rows, const int* __restrict__ top_of,
+                                                 const int* __restrict__ top_base, const int* __restrict__ top_ld,
+                                                 const long long* __restrict__ top_woff, const float* __restrict__ P,
+                                                 const float* __restrict__ v, float* __restrict__ out, const int* __restrict__ orig_of,
+                                                 float* __restrict__ zb, float* __restrict__ x, const int* __restrict__ sn_of,
+                                                 const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    const int q = blockIdx.x * 4 + (int)threadIdx.x / 64, lane = threadIdx.x & 63;
+    if (q >= nq) return;
+    const int i = rows[q];
+    if (i < 0 || sn_stopped(sn_prob, active, sn_of[i])) return;
+    const int k = top_of[q], base = top_base[k], ld = top_ld[k], r = q - base;
+    const float4* Pr = reinterpret_cast<const float4*>(P + top_woff[k] + (long long)r * ld);
+    const float4* vv = reinterpret_cast<const float4*>(v + base);
+    const int lo = kForward ? 0 : r, hi = kForward ? r : ld - 1;
+    float acc = 0.f;
+    for (int j = lo / 4 + lane; j <= hi / 4; j += 64) {
+        const float4 w = Pr[j], t = vv[j];
+        if (4 * j >= lo && 4 * j + 3 <= hi) acc += (w.x * t.x + w.y * t.y) + (w.z * t.z + w.w * t.w);
+        else acc += dot4_masked(w, t, 4 * j, lo, hi);
+    }
+    acc = group_sum<64>(acc);
+    if (lane == 0) {
+        if (kForward) {
+            out[q] = acc;
+        } else {
+            zb[i] = acc;
+            x[orig_of[i]] = acc;
+        }
+    }
+}
+
+// P (ld x ld row-major float, see k_top_tri) from Linv (m x m column-major double, lower)
+__global__ void k_top_pack(int m, int ld, const double* __restrict__ Li, float* __restrict__ P) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)m * ld) return;
+    const int i = (int)(e / ld), j = (int)(e % ld);
+    float val = 0.f;
+    if (j <= i) val = (float)Li[(long long)j * m + i];        // Linv(i, j)
+    else if (j < m) val = (float)Li[(long long)i * m + j];    // Linv(j, i)
+    P[(long long)i * ld + j] = val;
+}
+
 // dst[pos[i]] = src[i] and its inverse, for the combined systems' interface rows
 __global__ void k_pos_scatter(int n, const int* __restrict__ pos, const float* __restrict__ src, float* __restrict__ dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -514,9 +595,38 @@ static Csr transpose(const Csr& A) {
     return T;
 }
 
+static void rocblas_check(rocblas_status st, const char* what) {
+    if (st != rocblas_status_success) throw std::runtime_error(std::string("rocBLAS ") + what + ": " + rocblas_status_to_string(st));
+}
+
+// The block P of a dense top (see k_top_tri) from L_SS: the triangular inverse
+// by rocBLAS in double on the null stream, then rounded to float. One block at
+// a time (the device buffers are 2 m^2 doubles), serialised over the process's
+// threads.
+static void dense_top_inverse(const TriPlanHost::TopBlock& B, float* P) {
+    const int m = B.m;
+    DBuf<double> dL, dI;
+    {
+        std::vector<double> L;
+        dense_top_factor(B, L);
+        dL.upload(L);
+    }
+    dI.alloc((size_t)m * m);
+    static std::mutex mu;
+    static rocblas_handle handle = nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!handle) rocblas_check(rocblas_create_handle(&handle), "create handle");
+    rocblas_check(rocblas_set_stream(handle, nullptr), "set stream");
+    rocblas_check(rocblas_dtrtri(handle, rocblas_fill_lower, rocblas_diagonal_non_unit, m, dL.p, m, dI.p, m), "dtrtri");
+    const long long cells = (long long)m * B.ld;
+    hipLaunchKernelGGL(k_top_pack, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, nullptr, m, B.ld, (const double*)dI.p, P);
+    hdd_check(hipGetLastError(), "dense top pack");
+    hdd_check(hipStreamSynchronize(nullptr), "dense top");
+}
+
 void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, const std::vector<int>& prob_of) {
     TriPlanHost P;
-    build_triplan(F, offsets, P);
+    build_triplan(F, offsets, P, dense_top_min_rows());
     n = P.n; H = P.H; factor_nnz = P.factor_nnz;
     if (n == 0) return;
     {
@@ -540,12 +650,32 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
     diag.upload(P.diag); diagT.upload(P.diagT); l21.upload(P.l21);
     c.alloc(std::max<long long>(1, P.c_size)); bpart.alloc(std::max<long long>(1, P.bpart_size));
     t.alloc(n); y.alloc(2 * (size_t)n);  // forward result, then backward result
+    if (!P.tops.empty()) {
+        top_nq = (int)P.top_rows.size(); Gt = P.Gt;
+        std::vector<long long> block_off, woff;
+        long long wsize = 0;
+        top_dense_rows = 0;
+        for (const TriPlanHost::TopBlock& B : P.top_blocks) {
+            block_off.push_back(wsize);
+            wsize += (long long)B.ld * B.ld;
+            top_dense_rows += B.m;
+        }
+        std::vector<int> base, ld;
+        for (const TriPlanHost::Top& T : P.tops) { base.push_back(T.base); ld.push_back(T.ld); woff.push_back(block_off[T.w]); }
+        top_rows.upload(P.top_rows); top_of.upload(P.top_of); top_inc_ptr.upload(P.top_inc_ptr);
+        top_cpos.upload(P.top_cpos.empty() ? std::vector<long long>(1, 0) : P.top_cpos);
+        top_base.upload(base); top_ld.upload(ld); top_woff.upload(woff);
+        top_t.alloc(2 * (size_t)top_nq);  // t, then y (padding stays zero)
+        top_W.alloc(wsize);
+        for (size_t k = 0; k < P.top_blocks.size(); ++k) dense_top_inverse(P.top_blocks[k], top_W.p + block_off[k]);
+    }
 }
 
 size_t TriPlan::bytes() const {
     return (run_q.n + run_n.n + orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_prob.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
             ci_a1.n + bi_sn.n + bi_c.n + bi_j0.n) * 4 +
-           (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4;
+           (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4 +
+           (top_rows.n + top_of.n + top_inc_ptr.n + top_base.n + top_ld.n + top_W.n + top_t.n) * 4 + (top_cpos.n + top_woff.n) * 8;
 }
 
 #define HDD_SN_DISPATCH(G, KERNEL, BLOCKS, SHMEM, ...)                                                           \
@@ -657,9 +787,23 @@ void TriPlan::bwd_height(int h, float* x, hipStream_t s) {
     }
 }
 
+void TriPlan::top_solve(const float* b, float* x, hipStream_t s) {
+    float* tt = top_t.p;
+    float* ty = top_t.p + top_nq;
+    HDD_ROWS_DISPATCH(Gt, k_top_rhs, top_nq, s, top_nq, (const int*)top_rows.p, (const int*)top_inc_ptr.p, (const long long*)top_cpos.p,
+                      (const float*)c.p, (const int*)orig_of.p, b, tt, (const int*)sn_of.p, (const int*)sn_prob.p, active);
+    hipLaunchKernelGGL(k_top_tri<true>, dim3(blocks_for(top_nq, 4)), dim3(256), 0, s, top_nq, (const int*)top_rows.p, (const int*)top_of.p,
+                       (const int*)top_base.p, (const int*)top_ld.p, (const long long*)top_woff.p, (const float*)top_W.p,
+                       (const float*)tt, ty, (const int*)orig_of.p, y.p + n, x, (const int*)sn_of.p, (const int*)sn_prob.p, active);
+    hipLaunchKernelGGL(k_top_tri<false>, dim3(blocks_for(top_nq, 4)), dim3(256), 0, s, top_nq, (const int*)top_rows.p, (const int*)top_of.p,
+                       (const int*)top_base.p, (const int*)top_ld.p, (const long long*)top_woff.p, (const float*)top_W.p,
+                       (const float*)ty, (float*)nullptr, (const int*)orig_of.p, y.p + n, x, (const int*)sn_of.p, (const int*)sn_prob.p, active);
+}
+
 void TriPlan::solve(const float* b, float* x, hipStream_t s) {
     if (n == 0) return;
     for (int h = 0; h <= H; ++h) fwd_height(h, b, s);
+    if (top_nq) top_solve(b, x, s);
     for (int h = H; h >= 0; --h) bwd_height(h, x, s);
 }
 
@@ -920,6 +1064,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         L.Lii.build(fi, oi, pi);
         L.Lrr.build(fr, orr, prr);
         if (autotune()) S->tuned_choices += L.Lii.tune(stream) + L.Lrr.tune(stream);
+        S->dense_top_rows += L.Lii.top_dense_rows + L.Lrr.top_dense_rows;
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
         S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
@@ -966,6 +1111,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         }
         S->Lbottom.build(fb, ob, pb);
         if (autotune()) S->tuned_choices += S->Lbottom.tune(stream);
+        S->dense_top_rows += S->Lbottom.top_dense_rows;
         S->bot_rhs.alloc(boff); S->bot_sol.alloc(boff);
         S->bot_pos.upload(pos);
         S->bottom_k = m0[NP];

@@ -1,5 +1,7 @@
 #include "hdd_triplan.h"
 #include <algorithm>
+#include <climits>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <omp.h>
@@ -13,9 +15,72 @@ int lanes_for(double len) {
     while (g < 64 && g * 3 < len) g *= 2;
     return g;
 }
+
+// A height of a plan costs about 60 us in launches (forward and backward, about
+// 100 us measured on the big ones); a dense top of m rows reads m^2 floats per
+// solve (both triangles of its block) at about 4.5 TB/s, so a height joins the
+// tops while the blocks of all the plan's members grow by at most
+// 60e-6 * 4.5e12 / 4 B.
+constexpr long long kTopStepCells = 67500000;
+constexpr int kTopDefaultMinRows = 4096;
+constexpr int kTopMaxMembers = 64;  // plans of many factors share each height's launches: no dense tops
 }  // namespace
 
-void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P) {
+int dense_top_min_rows() {
+    const char* e = std::getenv("HDD_DENSE_TOP");
+    if (!e || !*e) return kTopDefaultMinRows;
+    return std::max(0, std::atoi(e));
+}
+
+int dense_top_cut(const SupernodalFactor& f, int top_min_rows, int members, int* m_out) {
+    *m_out = 0;
+    if (top_min_rows <= 0 || f.n < top_min_rows || f.sn.empty() || members < 1 || members > kTopMaxMembers) return INT_MAX;
+    std::vector<long long> rows(f.max_height + 1, 0);
+    for (const Supernode& S : f.sn) rows[S.height] += S.n;
+    long long m = 0;
+    int cut = INT_MAX;
+    for (int h = f.max_height; h >= 0; --h) {
+        const long long m2 = m + rows[h];
+        if (m2 > TriPlanHost::kTopMaxRows || 2 * m2 > f.n) break;
+        if ((m2 * m2 - m * m) * members > kTopStepCells) break;
+        m = m2;
+        cut = h;
+    }
+    *m_out = (int)m;
+    return cut;
+}
+
+void dense_top_factor(const TriPlanHost::TopBlock& B, std::vector<double>& L) {
+    const SupernodalFactor& f = *B.f;
+    const int m = B.m;
+    // S-local index of each permuted row of S: rows of S ascending
+    std::vector<int> loc(f.n, -1);
+    {
+        std::vector<char> in(f.n, 0);
+        for (const Supernode& S : f.sn)
+            if (S.height >= B.cut) for (int k = 0; k < S.n; ++k) in[S.s0 + k] = 1;
+        int q = 0;
+        for (int i = 0; i < f.n; ++i) if (in[i]) loc[i] = q++;
+        if (q != m) throw std::runtime_error("dense top: row count mismatch");
+    }
+    L.assign((size_t)m * m, 0.0);
+    for (size_t s = 0; s < f.sn.size(); ++s) {
+        const Supernode& S = f.sn[s];
+        if (S.height < B.cut) continue;
+        const int n = S.n, nr = (int)S.rows.size();
+        for (int j = 0; j < n; ++j) {
+            double* col = &L[(size_t)loc[S.s0 + j] * m];
+            for (int r = j; r < n; ++r) col[loc[S.s0 + r]] = f.L11[s][(size_t)j * n + r];
+            for (int a = 0; a < nr; ++a) {
+                const int li = loc[S.rows[a]];
+                if (li < 0) throw std::runtime_error("dense top: a row below the top is not in it");
+                col[li] = f.L21[s][(size_t)j * nr + a];
+            }
+        }
+    }
+}
+
+void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P, int top_min_rows) {
     P = TriPlanHost();
     for (size_t m = 0; m < F.size(); ++m) {
         P.n = std::max(P.n, offsets[m] + F[m]->n);
@@ -30,6 +95,11 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     std::vector<Ref> refs;
     std::vector<int> owner;  // per supernode: 1 if it holds the data (first member of its factor)
     std::map<const SupernodalFactor*, int> first_gid;  // factor -> global id of its first supernode, first member
+    std::map<const SupernodalFactor*, int> block_of;   // factor -> its dense top block
+    std::vector<char> in_top;                          // per supernode: 1 if in its factor's dense top
+    std::vector<int> top_member;                       // per top: factor index m
+    int top_members = 0;                               // factors big enough for a dense top
+    for (const SupernodalFactor* f : F) top_members += top_min_rows > 0 && f->n >= top_min_rows;
     long long dsize = 0, lsize = 0;
     for (size_t m = 0; m < F.size(); ++m) {
         const SupernodalFactor& f = *F[m];
@@ -37,6 +107,31 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         const auto seen = first_gid.find(&f);
         const int shared = seen == first_gid.end() ? -1 : seen->second;
         if (shared < 0) first_gid[&f] = (int)P.s0.size();
+        int cut = INT_MAX;
+        {
+            auto b = block_of.find(&f);
+            if (b == block_of.end()) {
+                int tm = 0;
+                const int c = dense_top_cut(f, top_min_rows, top_members, &tm);
+                int id = -1;
+                if (tm > 0) {
+                    id = (int)P.top_blocks.size();
+                    TriPlanHost::TopBlock B;
+                    B.f = &f; B.cut = c; B.m = tm;
+                    B.ld = (tm + TriPlanHost::kTopLd - 1) / TriPlanHost::kTopLd * TriPlanHost::kTopLd;
+                    P.top_blocks.push_back(B);
+                }
+                b = block_of.emplace(&f, id).first;
+            }
+            if (b->second >= 0) {
+                const TriPlanHost::TopBlock& B = P.top_blocks[b->second];
+                cut = B.cut;
+                TriPlanHost::Top T;
+                T.m = B.m; T.ld = B.ld; T.w = b->second;
+                P.tops.push_back(T);
+                top_member.push_back((int)m);
+            }
+        }
         for (int k = 0; k < f.n; ++k) P.orig_of[off + k] = off + f.perm[k];
         for (size_t s = 0; s < f.sn.size(); ++s) {
             const Supernode& S = f.sn[s];
@@ -55,6 +150,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
             }
             owner.push_back(shared < 0);
             height.push_back(S.height);
+            in_top.push_back(S.height >= cut);
             refs.push_back({(int)m, (int)s});
         }
     }
@@ -92,10 +188,45 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         for (int a = 0; a < P.nr[g]; ++a) ++P.inc_ptr[P.srows[P.roff[g] + a] + 1];
     for (int i = 0; i < P.n; ++i) P.inc_ptr[i + 1] += P.inc_ptr[i];
     P.cpos.assign(P.inc_ptr[P.n], 0);
+    std::vector<int> inc_src(P.inc_ptr[P.n], 0);  // incidence -> its supernode
     {
         std::vector<int> fill(P.inc_ptr.begin(), P.inc_ptr.end() - 1);
         for (int g = 0; g < nsn; ++g)
-            for (int a = 0; a < P.nr[g]; ++a) P.cpos[fill[P.srows[P.roff[g] + a]]++] = P.cofs[g] + a;
+            for (int a = 0; a < P.nr[g]; ++a) {
+                const int k = fill[P.srows[P.roff[g] + a]]++;
+                P.cpos[k] = P.cofs[g] + a;
+                inc_src[k] = g;
+            }
+    }
+    // dense tops: padded rows (ascending plan rows of S) and their incidences from R
+    {
+        std::vector<int> first_sn(F.size() + 1, 0);  // global supernode range of each factor member
+        for (size_t m = 0; m < F.size(); ++m) first_sn[m + 1] = first_sn[m] + (int)F[m]->sn.size();
+        P.top_inc_ptr.assign(1, 0);
+        double incs = 0, trows = 0;
+        for (size_t k = 0; k < P.tops.size(); ++k) {
+            TriPlanHost::Top& T = P.tops[k];
+            const int m = top_member[k];
+            T.base = (int)P.top_rows.size();
+            std::vector<int> rows;
+            for (int g = first_sn[m]; g < first_sn[m + 1]; ++g)
+                if (in_top[g]) for (int r = 0; r < P.sz[g]; ++r) rows.push_back(P.s0[g] + r);
+            std::sort(rows.begin(), rows.end());
+            if ((int)rows.size() != T.m) throw std::runtime_error("dense top: row count mismatch");
+            rows.resize(T.ld, -1);
+            for (int i : rows) {
+                P.top_rows.push_back(i);
+                P.top_of.push_back((int)k);
+                if (i >= 0) {
+                    for (int p = P.inc_ptr[i]; p < P.inc_ptr[i + 1]; ++p)
+                        if (!in_top[inc_src[p]]) P.top_cpos.push_back(P.cpos[p]);
+                    trows += 1;
+                }
+                P.top_inc_ptr.push_back((int)P.top_cpos.size());
+            }
+        }
+        incs = (double)P.top_cpos.size();
+        if (trows > 0) P.Gt = lanes_for(incs / trows);
     }
     // per height: supernodes, slices, rows, items
     const int nh = P.H + 1;
@@ -106,7 +237,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     P.sl_rows.assign(nh, TriPlanHost::kSliceRows);
     for (int h = 0; h < nh; ++h) {
         int mx = 0;
-        for (int g = 0; g < nsn; ++g) if (height[g] == h) mx = std::max(mx, P.sz[g]);
+        for (int g = 0; g < nsn; ++g) if (height[g] == h && !in_top[g]) mx = std::max(mx, P.sz[g]);
         if (mx > TriPlanHost::kSliceRows) P.sl_rows[h] = TriPlanHost::kBigSliceRows;
     }
     for (int h = 0; h < nh; ++h) {
@@ -115,7 +246,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         P.bi_begin[h] = (int)P.bi_sn.size();
         double rows = 0, diag_len = 0, contrib_rows = 0, contrib_len = 0, incs = 0, chunks = 0;
         std::vector<int> at_h;
-        for (int g = 0; g < nsn; ++g) if (height[g] == h) at_h.push_back(g);
+        for (int g = 0; g < nsn; ++g) if (height[g] == h && !in_top[g]) at_h.push_back(g);
         // supernodes that share blocks run next to each other (the shared blocks stay in cache)
         std::stable_sort(at_h.begin(), at_h.end(), [&](int a, int b) { return P.loff[a] != P.loff[b] ? P.loff[a] < P.loff[b] : P.doff[a] < P.doff[b]; });
         for (int g : at_h) {
@@ -183,63 +314,6 @@ void build_slices_items(const std::vector<int>& sn_list, const std::vector<int>&
     }
     sl_begin[nh] = (int)sl_sn.size();
     ci_begin[nh] = (int)ci_sn.size();
-}
-
-void triplan_solve_host(const TriPlanHost& P, const float* b, float* x) {
-    if (P.n == 0) return;
-    std::vector<double> y(P.n, 0.0), z(P.n, 0.0), c(P.c_size, 0.0), bp(P.bpart_size, 0.0), t(P.n, 0.0);
-    for (int h = 0; h <= P.H; ++h) {
-        for (int q = P.row_begin[h]; q < P.row_begin[h + 1]; ++q) {
-            const int i = P.rowlist[q];
-            double s = 0;
-            for (int p = P.inc_ptr[i]; p < P.inc_ptr[i + 1]; ++p) s += c[P.cpos[p]];
-            t[i] = (double)b[P.orig_of[i]] - s;
-        }
-        for (int q = P.sn_begin[h]; q < P.sn_begin[h + 1]; ++q) {
-            const int g = P.sn_list[q], n = P.sz[g], base = P.s0[g];
-            const float* D = &P.diag[P.doff[g]];
-            for (int r = 0; r < n; ++r) {
-                double acc = 0;
-                for (int k = 0; k <= r; ++k) acc += (double)D[(size_t)r * n + k] * t[base + k];
-                y[base + r] = acc;
-            }
-        }
-        for (int it = P.ci_begin[h]; it < P.ci_begin[h + 1]; ++it) {
-            const int g = P.ci_sn[it], n = P.sz[g];
-            for (int a = P.ci_a0[it]; a < P.ci_a1[it]; ++a) {
-                double acc = 0;
-                for (int j = 0; j < n; ++j) acc += (double)P.l21[P.loff[g] + (long long)a * n + j] * y[P.s0[g] + j];
-                c[P.cofs[g] + a] = acc;
-            }
-        }
-    }
-    for (int h = P.H; h >= 0; --h) {
-        for (int it = P.bi_begin[h]; it < P.bi_begin[h + 1]; ++it) {
-            const int g = P.bi_sn[it], n = P.sz[g], cc = P.bi_c[it], j0 = P.bi_j0[it];
-            const int a0 = cc * TriPlanHost::kTile, a1 = std::min(P.nr[g], a0 + TriPlanHost::kTile);
-            for (int j = j0; j < std::min(n, j0 + TriPlanHost::kTile); ++j) {
-                double acc = 0;
-                for (int a = a0; a < a1; ++a) acc += (double)P.l21[P.loff[g] + (long long)a * n + j] * z[P.srows[P.roff[g] + a]];
-                bp[P.bofs[g] + (long long)cc * n + j] = acc;
-            }
-        }
-        for (int q = P.sn_begin[h]; q < P.sn_begin[h + 1]; ++q) {
-            const int g = P.sn_list[q], n = P.sz[g], base = P.s0[g];
-            const int nch = (P.nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
-            for (int k = 0; k < n; ++k) {
-                double s = 0;
-                for (int cc = 0; cc < nch; ++cc) s += bp[P.bofs[g] + (long long)cc * n + k];
-                t[base + k] = y[base + k] - s;
-            }
-            const float* DT = &P.diagT[P.doff[g]];
-            for (int r = 0; r < n; ++r) {
-                double acc = 0;
-                for (int k = r; k < n; ++k) acc += (double)DT[(size_t)r * n + k] * t[base + k];
-                z[base + r] = acc;
-                x[P.orig_of[base + r]] = (float)acc;
-            }
-        }
-    }
 }
 
 }  // namespace hdd

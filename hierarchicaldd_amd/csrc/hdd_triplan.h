@@ -70,9 +70,56 @@ struct TriPlanHost {
     std::vector<int> Gb;                      // lanes per row of the backward reduce, per height
     long long bpart_size = 0;
     size_t factor_nnz = 0;
+    // Dense tops. The top heights of a tall tree hold few supernodes each, so
+    // every height costs a handful of launches with little parallelism. For a
+    // factor of at least top_min_rows rows, the supernodes S of its top heights
+    // (ancestor-closed, at most kTopMaxRows rows) leave the per-height lists and
+    // their part of the solve becomes two dense triangular matvecs:
+    //   t_S = b_S - L_SR y_R          (contributions of the rest, R, to S rows)
+    //   y_S = Linv t_S,  z_S = Linv^T y_S,  Linv = L_SS^{-1}
+    // run between the forward and the backward pass (z_S feeds the backward
+    // tiles of R). Linv is formed on the device at set-up from L_SS in double
+    // and kept in float, as the diagonal blocks' inverses are; the inverse of
+    // L_SS L_SS^T itself would lose eps * cond instead of eps * sqrt(cond).
+    static constexpr int kTopMaxRows = 12288;
+    static constexpr int kTopLd = 64;         // row stride of the dense blocks and of t: a multiple of this
+    struct Top {
+        int m = 0, ld = 0;    // rows of S, padded row stride
+        int base = 0;         // first entry of the top in top_rows (padded rows)
+        int w = 0;            // dense block: tops of a repeated factor share one
+    };
+    struct TopBlock {
+        const SupernodalFactor* f = nullptr;  // factor the block is assembled from (valid while F is)
+        int cut = 0;                          // supernodes of f at heights >= cut form S
+        int m = 0, ld = 0;
+    };
+    std::vector<Top> tops;
+    std::vector<TopBlock> top_blocks;
+    std::vector<int> top_rows;                // per padded top row: plan row (-1 for padding), tops in order
+    std::vector<int> top_of;                  // per padded top row: its top
+    std::vector<int> top_inc_ptr;             // per padded top row: incidences from R, [ptr[q], ptr[q+1])
+    std::vector<long long> top_cpos;          // incidence -> position in c (ascending supernode)
+    int Gt = 4;                               // lanes per row of the top's right-hand side
 };
 
-void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P);
+// top_min_rows: factors with at least this many rows get a dense top (0: none).
+void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P, int top_min_rows = 0);
+
+// The height from which the supernodes of f form its dense top, with the top's
+// rows in *m (a height above f's tree and *m = 0 when f gets none), in a plan
+// with `members` factors of at least top_min_rows rows (at most 64). Heights
+// are taken from the root down while the top stays within kTopMaxRows rows and
+// half of f, and while the dense blocks of all members grow by less than one
+// height's launches cost.
+int dense_top_cut(const SupernodalFactor& f, int top_min_rows, int members, int* m);
+
+// Dense L_SS of a top block, m x m column-major in double (rows of S in
+// ascending permuted order; S is ancestor-closed, so L_SS is lower triangular).
+void dense_top_factor(const TriPlanHost::TopBlock& B, std::vector<double>& L);
+
+// Rows a factor needs to have a dense top by default, or HDD_DENSE_TOP from the
+// environment (0 turns dense tops off).
+int dense_top_min_rows();
 
 // (Re)builds the row slices of the diagonal steps and the items of the forward
 // contributions from sn_list, sl_rows and ci_entries. Every row's sum is the
@@ -81,9 +128,5 @@ void build_slices_items(const std::vector<int>& sn_list, const std::vector<int>&
                         const std::vector<int>& nr, const std::vector<int>& sl_rows, const std::vector<int>& ci_entries,
                         std::vector<int>& sl_sn, std::vector<int>& sl_r0, std::vector<int>& sl_begin, std::vector<int>& ci_sn,
                         std::vector<int>& ci_a0, std::vector<int>& ci_a1, std::vector<int>& ci_begin);
-
-// Host execution of the plan (double accumulation over its float data), for
-// tests of the plan construction against the factor's own solve.
-void triplan_solve_host(const TriPlanHost& P, const float* b, float* x);
 
 }  // namespace hdd

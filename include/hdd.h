@@ -221,8 +221,12 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * evaluation against the 200 smooths on a random right-hand side (-1: not run),
  * out[13] launch shapes of the triangular-solve plans that the setup's
  * autotuner changed from their defaults (HDD_AUTOTUNE=0 in the environment
- * turns it off; results do not depend on it).
- * n = number of slots provided (up to 14). No reference counterpart. */
+ * turns it off; results do not depend on it), out[14] rows of the dense top
+ * blocks of the triangular-solve plans (each tall tree's top heights solved as
+ * two dense triangular matvecs; HDD_DENSE_TOP=0 in the environment turns them
+ * off, HDD_DENSE_TOP=k gives a dense top to factors of at least k rows, 4096
+ * by default).
+ * n = number of slots provided (up to 15). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
 /* Measurement: runs `reps` exact solves of one triangular-solve plan of sigma
