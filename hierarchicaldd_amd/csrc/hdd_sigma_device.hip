@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_reduce(const int* __restrict__ 
     const int i = rows[w];
     if (sn_stopped(sn_prob, active, sn_of[i])) return;
     float s = 0.f;
+#pragma unroll 8
     for (int p = inc_ptr[i] + lane; p < inc_ptr[i + 1]; p += G) s += c[cpos[p]];
     s = group_sum<G>(s);
     if (lane == 0) t[i] = b[orig_of[i]] - s;
@@ -91,6 +92,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, cons
             ts[k] = tg[i];
         } else {
             float sum = 0.f;
+#pragma unroll 8
             for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
             ts[k] = b[orig_of[i]] - sum;
         }
@@ -101,6 +103,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, cons
     for (int r = r0 + (int)threadIdx.x / G; r < rend; r += 256 / G) {
         const float* Dr = D + (long long)r * n;
         float acc = 0.f;
+#pragma unroll 8
         for (int k = lane; k <= r; k += G) acc += Dr[k] * ts[k];
         acc = group_sum<G>(acc);
         if (lane == 0) y[base + r] = acc;
@@ -126,6 +129,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_contrib(int i0, int i1, const i
     for (int a = ci_a0[it] + (int)threadIdx.x / G; a < a1; a += 256 / G) {
         const float* La = L + (long long)a * n;
         float acc = 0.f;
+#pragma unroll 8
         for (int j = lane; j < n; j += G) acc += La[j] * yy[j];
         acc = group_sum<G>(acc);
         if (lane == 0) cg[a] = acc;
@@ -146,12 +150,17 @@ __global__ void __launch_bounds__(256) k_tri_bwd_tiles(int i0, int i1, const int
     const int g = bi_sn[it], n = sz[g], cc = bi_c[it], j = bi_j0[it] + lane;
     if (sn_stopped(sn_prob, active, g)) return;
     const int a0 = cc * TriPlanHost::kTile, m = min(nr[g] - a0, TriPlanHost::kTile);
-    // lane q holds z at the tile's row q; the row loop then issues only independent L21 loads
+    // lane q holds z at the tile's row q; all the tile's L21 loads are issued
+    // before the first product (one memory round trip), then summed in row order
     const float zq = lane < m ? z[srows[roff[g] + a0 + lane]] : 0.f;
     const float* L = l21 + loff[g] + (long long)a0 * n + min(j, n - 1);
+    float lv[TriPlanHost::kTile];
+#pragma unroll
+    for (int q = 0; q < TriPlanHost::kTile; ++q) lv[q] = q < m ? L[(long long)q * n] : 0.f;
     float acc = 0.f;
-#pragma unroll 8
-    for (int q = 0; q < m; ++q) acc += L[(long long)q * n] * __shfl(zq, q);
+#pragma unroll
+    for (int q = 0; q < TriPlanHost::kTile; ++q)
+        if (q < m) acc += lv[q] * __shfl(zq, q);
     if (j < n) bpart[bofs[g] + (long long)cc * n + j] = acc;
 }
 
@@ -170,6 +179,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_reduce(const int* __restrict__ 
     const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
     const float* P = bpart + bofs[g] + k;
     float s = 0.f;
+#pragma unroll 8
     for (int cc = lane; cc < nch; cc += G) s += P[(long long)cc * n];
     s = group_sum<G>(s);
     if (lane == 0) t[i] = y[i] - s;
@@ -199,6 +209,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
         } else {
             const float* P = bpart + bofs[g] + k;
             float sum = 0.f;
+#pragma unroll 8
             for (int cc = 0; cc < nch; ++cc) sum += P[(long long)cc * n];
             ts[k] = y[base + k] - sum;
         }
@@ -209,6 +220,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
     for (int r = r0 + (int)threadIdx.x / G; r < rend; r += 256 / G) {
         const float* Dr = DT + (long long)r * n;
         float acc = 0.f;
+#pragma unroll 8
         for (int k = r + lane; k < n; k += G) acc += Dr[k] * ts[k];
         acc = group_sum<G>(acc);
         if (lane == 0) { z[base + r] = acc; x[orig_of[base + r]] = acc; }
@@ -244,6 +256,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
         const int j = idx / n, k = idx - j * n;
         const int i = base_s[j] + k;
         float sum = 0.f;
+#pragma unroll 8
         for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
         ts[j][k] = b[orig_of[i]] - sum;
     }
@@ -498,6 +511,7 @@ __global__ void __launch_bounds__(256) k_top_rhs(int nq, const int* __restrict__
     const int i = rows[q];
     if (i < 0 || sn_stopped(sn_prob, active, sn_of[i])) return;
     float s = 0.f;
+#pragma unroll 8
     for (int p = inc_ptr[q] + lane; p < inc_ptr[q + 1]; p += G) s += c[cpos[p]];
     s = group_sum<G>(s);
     if (lane == 0) t[q] = b[orig_of[i]] - s;
@@ -527,8 +541,9 @@ __global__ void __launch_bounds__(256) k_top_tri(int nq, const int* __restrict__
                                                  const float* __restrict__ v, float* __restrict__ out, const int* __restrict__ orig_of,
                                                  float* __restrict__ zb, float* __restrict__ x, const int* __restrict__ sn_of,
                                                  const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    const int q = blockIdx.x * 4 + (int)threadIdx.x / 64, lane = threadIdx.x & 63;
-    if (q >= nq) return;
+    const int qq = blockIdx.x * 4 + (int)threadIdx.x / 64, lane = threadIdx.x & 63;
+    if (qq >= nq) return;
+    const int q = kForward ? nq - 1 - qq : qq;  // longest rows first (forward rows grow with r, backward rows shrink)
     const int i = rows[q];
     if (i < 0 || sn_stopped(sn_prob, active, sn_of[i])) return;
     const int k = top_of[q], base = top_base[k], ld = top_ld[k], r = q - base;
