@@ -243,7 +243,9 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
                                                        const long long* __restrict__ loff, const long long* __restrict__ cofs,
                                                        const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    __shared__ float ts[M][TriPlanHost::kSliceRows], ys[M][TriPlanHost::kSliceRows];
+    // member-minor vectors: one row's values of all members are one (M = 8: two 16-byte) LDS read
+    __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
+    __shared__ __align__(16) float ys[TriPlanHost::kSliceRows][M];
     __shared__ int base_s[M];
     const int q0 = run_q[r0 + blockIdx.x], cnt = run_n[r0 + blockIdx.x];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
@@ -252,13 +254,17 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
     if (!any) return;
     if (threadIdx.x < cnt) base_s[threadIdx.x] = s0[sn_list[q0 + threadIdx.x]];
     __syncthreads();
-    for (int idx = threadIdx.x; idx < cnt * n; idx += 256) {
+    for (int idx = threadIdx.x; idx < M * n; idx += 256) {
         const int j = idx / n, k = idx - j * n;
-        const int i = base_s[j] + k;
-        float sum = 0.f;
+        float v = 0.f;  // members past the run's count stay zero
+        if (j < cnt) {
+            const int i = base_s[j] + k;
+            float sum = 0.f;
 #pragma unroll 8
-        for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
-        ts[j][k] = b[orig_of[i]] - sum;
+            for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
+            v = b[orig_of[i]] - sum;
+        }
+        ts[k][j] = v;
     }
     __syncthreads();
     const float* D = diag + doff[g0];
@@ -274,16 +280,21 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
             const int k = lane + u * G;
             if (k <= r) {
                 const float d = Dr[k];
+                float tk[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
 #pragma unroll
                 for (int j = 0; j < M; ++j)
-                    if (j < cnt) acc[j] += d * ts[j][k];
+                    if (j < cnt) acc[j] += d * tk[j];
             }
         }
 #pragma unroll
         for (int j = 0; j < M; ++j)
             if (j < cnt) {
                 const float v = group_sum<G>(acc[j]);
-                if (lane == 0) { ys[j][r] = v; y[base_s[j] + r] = v; }
+                if (lane == 0) { ys[r][j] = v; y[base_s[j] + r] = v; }
+            } else if (lane == 0) {
+                ys[r][j] = 0.f;
             }
     }
     __syncthreads();
@@ -298,9 +309,12 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
             const int k = lane + u * G;
             if (k < n) {
                 const float l = La[k];
+                float yk[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) yk[j] = ys[k][j];
 #pragma unroll
                 for (int j = 0; j < M; ++j)
-                    if (j < cnt) acc[j] += l * ys[j][k];
+                    if (j < cnt) acc[j] += l * yk[j];
             }
         }
 #pragma unroll
@@ -326,9 +340,9 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
                                                        float* __restrict__ z, float* __restrict__ x,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     constexpr int kChunk = 128;  // a multiple of 4: the rows' order per wavefront does not depend on it
-    __shared__ float zr[M][kChunk];
+    __shared__ __align__(16) float zr[kChunk][M];  // member-minor, as ts
     __shared__ float part[4][M][TriPlanHost::kSliceRows];
-    __shared__ float ts[M][TriPlanHost::kSliceRows];
+    __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
     __shared__ int base_s[M], roff_s[M];
     const int q0 = run_q[r0 + blockIdx.x], cnt = run_n[r0 + blockIdx.x];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
@@ -345,21 +359,23 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
     __syncthreads();
     for (int a0 = 0; a0 < m; a0 += kChunk) {
         const int cm = min(kChunk, m - a0);
-        for (int idx = threadIdx.x; idx < cnt * cm; idx += 256) {
+        for (int idx = threadIdx.x; idx < M * cm; idx += 256) {
             const int j = idx / cm, q = idx - j * cm;
-            zr[j][q] = z[srows[roff_s[j] + a0 + q]];
+            zr[q][j] = j < cnt ? z[srows[roff_s[j] + a0 + q]] : 0.f;
         }
         __syncthreads();
 #pragma unroll 4
         for (int q = w; q < cm; q += 4) {
             const float* La = L + (long long)(a0 + q) * n;
             const float l0 = La[j0], l1 = La[j1];
+            float zq[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) zq[j] = zr[q][j];
 #pragma unroll
             for (int j = 0; j < M; ++j)
                 if (j < cnt) {
-                    const float zq = zr[j][q];
-                    acc0[j] += l0 * zq;
-                    acc1[j] += l1 * zq;
+                    acc0[j] += l0 * zq[j];
+                    acc1[j] += l1 * zq[j];
                 }
         }
         __syncthreads();
@@ -371,9 +387,9 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
             if (lane + 64 < n) part[w][j][lane + 64] = acc1[j];
         }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < cnt * n; idx += 256) {
+    for (int idx = threadIdx.x; idx < M * n; idx += 256) {
         const int j = idx / n, k = idx - j * n;
-        ts[j][k] = y[base_s[j] + k] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k]));
+        ts[k][j] = j < cnt ? y[base_s[j] + k] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k])) : 0.f;
     }
     __syncthreads();
     const float* DT = diagT + doff[g0];
@@ -389,9 +405,12 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
             const int k = r + gl + u * G;
             if (k < n) {
                 const float d = Dr[k];
+                float tk[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
 #pragma unroll
                 for (int j = 0; j < M; ++j)
-                    if (j < cnt) acc[j] += d * ts[j][k];
+                    if (j < cnt) acc[j] += d * tk[j];
             }
         }
 #pragma unroll
