@@ -380,13 +380,13 @@ hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
     const SigmaDevice* G = S.sigma.get();
     int nsub = 0;
     for (auto& q : S.prob) nsub += q->sigma.nsub;
-    double v[15] = {S.setup_seconds, G ? G->factor_seconds : 0.0, G ? (double)G->plan_bytes : 0.0,
+    double v[16] = {S.setup_seconds, G ? G->factor_seconds : 0.0, G ? (double)G->plan_bytes : 0.0,
                     (double)S.lv[0].host.npages(), (G && G->levels) ? (double)G->lv[0].n_if : (S.sigma_host().levels ? (double)S.sigma_host().n_if[0] : 0.0),
                     (G && G->levels) ? (double)G->lv[0].n_int : 0.0, (double)nsub,
                     G ? G->bottom_rho : -1.0, G ? (double)G->bottom_direct : 0.0, G ? (double)G->bottom_k : 0.0,
                     G ? G->bottom_seconds : 0.0, G ? (double)G->bottom_steps : 0.0, G ? G->bottom_check : -1.0,
-                    G ? (double)G->tuned_choices : 0.0, G ? (double)G->dense_top_rows : 0.0};
-    for (int i = 0; i < n && i < 15; ++i) out[i] = v[i];
+                    G ? (double)G->tuned_choices : 0.0, G ? (double)G->dense_top_rows : 0.0, S.bottom_local ? 1.0 : 0.0};
+    for (int i = 0; i < n && i < 16; ++i) out[i] = v[i];
     return HDD_OK;
 }
 

@@ -35,6 +35,13 @@ void upload_constants();
 // V-cycle kernels, float storage and half (FP16) storage with float arithmetic
 void launch_smooth(const DevLevel& L, const float* u_in, float* u_out, const float* b, bool interior, bool boundary, hipStream_t s);
 void launch_smooth(const DevLevel& L, const __half* u_in, __half* u_out, const __half* b, bool interior, bool boundary, hipStream_t s);
+// `count` sweeps (interior and boundary) of a level whose Active cells couple only within clusters of at most
+// four pages, in one launch: nwg workgroups of four page slots (wg_page, -1: empty), nb_slot[p*6+f] the slot
+// of page p's face-f neighbour when it is in p's cluster, else -1
+void launch_smooth_local(const DevLevel& L, const int* wg_page, int nwg, const int* nb_slot, const float* u_in, float* u_out,
+                         const float* b, int count, hipStream_t s);
+void launch_smooth_local(const DevLevel& L, const int* wg_page, int nwg, const int* nb_slot, const __half* u_in, __half* u_out,
+                         const __half* b, int count, hipStream_t s);
 void launch_residual(const DevLevel& L, const float* u, const float* b, float* r, hipStream_t s);
 void launch_residual(const DevLevel& L, const __half* u, const __half* b, __half* r, hipStream_t s);
 void launch_fill(float* a, float v, size_t n, hipStream_t s);

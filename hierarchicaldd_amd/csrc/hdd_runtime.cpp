@@ -165,6 +165,67 @@ void Solver::merge_levels() {
     }
 }
 
+// Page-by-page sweeps of a level (the V-cycle bottom): pages are linked when
+// an Active cell has a set face bit to an Active cell of the other page;
+// linked pages form clusters. When no cluster has more than four pages, whole
+// clusters are packed into workgroups of four page slots (wg_page, -1 for an
+// empty slot) and nb_slot[p*6+f] gives the slot of page p's face-f neighbour
+// page when it is in p's cluster (else -1). Every other face neighbour then
+// holds no cell that a coupled Active cell reads and a sweep writes. Returns
+// false (sweeps stay global) when a cluster is larger.
+static bool plan_local_sweeps(const PagedLevel& H, std::vector<int>& wg_page, std::vector<int>& nb_slot) {
+    static const int d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    const int np = H.npages();
+    std::vector<int> parent(np);
+    for (int p = 0; p < np; ++p) parent[p] = p;
+    auto find = [&](int a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+    };
+    for (int p = 0; p < np; ++p) {
+        const Int3 o = H.page_base[p];
+        for (int c = 0; c < kPageCells; ++c) {
+            const uint32_t f = H.flags[(size_t)p * kPageCells + c];
+            if (!(f & kActive)) continue;
+            const int x = o.x + (c >> 6), y = o.y + ((c >> 3) & 7), z = o.z + (c & 7);
+            for (int q = 0; q < 6; ++q) {
+                if (!(f & (kFaceMX << q))) continue;
+                const int nx = x + d[q][0], ny = y + d[q][1], nz = z + d[q][2];
+                const int pn = H.page_at(nx, ny, nz);
+                if (pn == p || pn < 0 || !(H.flag(nx, ny, nz) & kActive)) continue;
+                parent[find(pn)] = find(p);
+            }
+        }
+    }
+    std::vector<std::vector<int>> members(np);
+    for (int p = 0; p < np; ++p) members[find(p)].push_back(p);  // ascending pages per cluster
+    for (int r = 0; r < np; ++r)
+        if (members[r].size() > 4) return false;
+    wg_page.clear();
+    std::vector<int> slot(np, -1), wg_of(np, -1);
+    int used = 4;
+    for (int p = 0; p < np; ++p) {
+        const std::vector<int>& m = members[find(p)];
+        if (m.front() != p) continue;  // clusters in the order of their first page (Morton order keeps them local)
+        if (used + (int)m.size() > 4) { wg_page.insert(wg_page.end(), 4, -1); used = 0; }
+        for (int q : m) {
+            slot[q] = used;
+            wg_of[q] = (int)wg_page.size() / 4 - 1;
+            wg_page[wg_page.size() - 4 + used] = q;
+            ++used;
+        }
+    }
+    nb_slot.assign((size_t)np * 6, -1);
+    for (int p = 0; p < np; ++p) {
+        const Int3 o = H.page_base[p];
+        for (int q = 0; q < 6; ++q) {
+            const int pn = H.page_at(o.x + d[q][0] * kPX, o.y + d[q][1] * kPY, o.z + d[q][2] * kPZ);
+            if (pn >= 0 && find(pn) == find(p) && wg_of[pn] == wg_of[p]) nb_slot[(size_t)p * 6 + q] = slot[pn];
+        }
+    }
+    return true;
+}
+
 void Solver::build_device() {
     const int NP = nprob();
     for (auto& q : prob) q->factorize();
@@ -173,6 +234,12 @@ void Solver::build_device() {
     upload_constants();
     merge_levels();
     upload_levels();
+    {
+        const char* e = std::getenv("HDD_BOTTOM_LOCAL");
+        std::vector<int> wp, ns;
+        bottom_local = !(e && std::string(e) == "0") && plan_local_sweeps(lv.back().host, wp, ns);
+        if (bottom_local) { bottom_wg_page.upload(wp); bottom_nb_slot.upload(ns); }
+    }
     for (int l = 0; l < (int)lv.size(); ++l) {
         std::vector<int> pp((size_t)lv[l].host.npages() + 1, -1);
         for (int p = 0; p < NP; ++p)
@@ -316,7 +383,15 @@ void Solver::vcycle_t() {
         launch_fill(V::u(lv[l + 1], lv[l + 1].cur), 0.f, lv[l + 1].cells(), stream);
         launch_restrict(lv[l].view(), lv[l + 1].view(), V::r(lv[l]), V::b(lv[l + 1]), stream);
     }
-    sweeps_t<S>(L - 1, 200, true, true);
+    if (bottom_local) {
+        // one launch; the result goes to the other buffer (other workgroups read this one's neighbour pages)
+        LevelBuffers& B = lv[L - 1];
+        launch_smooth_local(B.view(), bottom_wg_page.p, (int)(bottom_wg_page.n / 4), bottom_nb_slot.p, V::u(B, B.cur), V::u(B, 1 - B.cur),
+                            V::b(B), 200, stream);
+        B.cur = 1 - B.cur;
+    } else {
+        sweeps_t<S>(L - 1, 200, true, true);
+    }
     for (int l = L - 2; l >= 0; --l) {
         launch_prolong(lv[l].view(), lv[l + 1].view(), V::u(lv[l], lv[l].cur), V::u(lv[l + 1], lv[l + 1].cur), stream);
         sweeps_t<S>(l, P.bnd_s, false, true);

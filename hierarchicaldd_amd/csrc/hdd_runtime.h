@@ -127,6 +127,11 @@ public:
     bool use_graph = false;            // replay the preconditioner apply from captured HIP graphs (opt-in)
     bool half_storage = false;         // V-cycles store u, b, r in FP16 (FP32 arithmetic); z then lives in zf
     DBuf<float> zf;                    // preconditioner output of the FP16-storage variant
+    // the coarsest level's 200 sweeps run in one launch, a workgroup per four
+    // page slots of whole page clusters (its Active cells couple only within
+    // clusters of at most four pages; HDD_BOTTOM_LOCAL=0 turns it off)
+    bool bottom_local = false;
+    DBuf<int> bottom_wg_page, bottom_nb_slot;
 
     // one problem; host_only: run the host setup and skip every device allocation (introspection)
     Solver(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b,

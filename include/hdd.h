@@ -225,8 +225,11 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * blocks of the triangular-solve plans (each tall tree's top heights solved as
  * two dense triangular matvecs; HDD_DENSE_TOP=0 in the environment turns them
  * off, HDD_DENSE_TOP=k gives a dense top to factors of at least k rows, 4096
- * by default).
- * n = number of slots provided (up to 15). No reference counterpart. */
+ * by default), out[15] 1 if the interior V-cycle's 200 coarsest-level sweeps
+ * run page by page in one launch (bit-identical to 200 launches; set-up picks
+ * it when that level's Active cells couple only within their pages,
+ * HDD_BOTTOM_LOCAL=0 turns it off), else 0.
+ * n = number of slots provided (up to 16). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
 /* Measurement: runs `reps` exact solves of one triangular-solve plan of sigma

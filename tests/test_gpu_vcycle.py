@@ -79,3 +79,46 @@ def test_vcycle_linear_and_zero(pair):
     z1 = g.vcycle(b)
     z2 = g.vcycle(2 * b)
     assert same_bits(2 * z1, z2)
+
+
+LOCAL_BOTTOM_CASES = {
+    # every subdomain's coarsest block inside one page
+    "sphere64_sub16": domain.config0,
+    # 32-cell subdomains, 4 levels: one cluster of two coupled pages at the coarsest level
+    "sphere96_sub32": lambda: (domain.sphere_obstacle_box(96, [(48, 48, 48, 24)], delta=((24, 24, 24), 1.0)),
+                               domain.DDConfig(sub_size=32, vlevels=4)),
+}
+
+
+@pytest.mark.parametrize("fp16", [False, True])
+@pytest.mark.parametrize("name", list(LOCAL_BOTTOM_CASES))
+def test_bottom_page_local_bitwise(name, fp16):
+    """The coarsest level's 200 sweeps run in one launch, a workgroup per
+    cluster of coupled pages, when no cluster has more than four pages. Results
+    must equal those of the 200 sweep launches bit for bit (HDD_BOTTOM_LOCAL=0),
+    in FP32 and in FP16 storage, for preconditioner applications and whole PCG
+    solves."""
+    import os
+    from hierarchicaldd_amd import hdd
+    dom, cfg = LOCAL_BOTTOM_CASES[name]()
+    tol = 1e-6 * float(np.abs(dom.b).max())
+    out = {}
+    for mode in ("0", ""):
+        os.environ["HDD_BOTTOM_LOCAL"] = mode
+        try:
+            g = hdd.DDSolver(dom, cfg)
+        finally:
+            os.environ.pop("HDD_BOTTOM_LOCAL", None)
+        try:
+            g.set_fp16(fp16)
+            b = random_rhs(g.level(0)[1], 31)
+            z = g.apply_preconditioner(b)
+            it, hist, _ = g.solve(tol, 200)
+            out[mode] = (g.stats()["bottom_page_local"], z, it, hist, g.get(hdd.CH_X))
+        finally:
+            g.close()
+    (l0, z0, i0, h0, x0), (l1, z1, i1, h1, x1) = out["0"], out[""]
+    print(f"[{name} fp16={fp16}] page-local bottom {l1:.0f}; iterations {i0} / {i1}")
+    assert l0 == 0 and l1 == 1
+    assert same_bits(z0, z1)
+    assert i0 == i1 and np.array_equal(h0, h1) and same_bits(x0, x1)
