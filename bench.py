@@ -9,7 +9,9 @@ plan resident in HBM before the timed region.
 
 Printed JSON line: value = solves/s (1 / time-to-tolerance); the roofline
 object covers the level-0 damped-Jacobi sweep kernel, timed with HIP events
-on the context stream; cpu_baseline is the oracle (the CPU restatement of the
+on the context stream; "sigma_solves" gives the same for one exact solve of
+each triangular-solve plan of the interface solver (time per solve, bytes of
+the factor entries read); cpu_baseline is the oracle (the CPU restatement of the
 reference dd_pcg) on a bounded sample: the complete 64^3 problem (setup +
 solve), set beside the GPU's rate on the same 64^3 problem.
 
@@ -117,6 +119,19 @@ def main():
     achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic, {achieved:.0f} GB/s")
 
+    # exact solves of the interface solver's triangular-solve plans (the kernels that take most of a
+    # PCG iteration): mean time per solve with HIP events, algorithmic bytes = 4 B per factor entry
+    # per pass, two passes (subdomains that share one factor counted each)
+    sigma_solves = []
+    n_sigma, _ = solver.sigma_info()
+    plans = [(l, w) for l in range(n_sigma) for w in ("Aii", "Arr")] + ([(n_sigma - 1, "bottom")] if st["bottom_direct"] else [])
+    for l, w in plans:
+        ms, by = solver.profile_solve(l, w, 20)
+        gbs = by / (ms * 1e-3) / 1e9
+        sigma_solves.append({"level": l, "plan": w, "us_per_solve": ms * 1e3, "algorithmic_MB": by / 1e6,
+                             "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
+        log(f"[bench] sigma level {l} {w} solve: {ms * 1e3:.0f} us, {by / 1e6:.0f} MB algorithmic, {gbs:.0f} GB/s")
+
     # FP16-storage V-cycles (FP32 arithmetic) on the same problem
     solver.set_fp16(True)
     solver.solve(tol, args.max_iterations)
@@ -161,6 +176,7 @@ def main():
         "roofline": {"kernel": "k_smooth (level-0 damped-Jacobi sweep)", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": counter_traffic(dom.n_pages) if args.config == "sphere256" else None},
+        "sigma_solves": sigma_solves,
         "cpu_baseline": cpu,
         "fp16_storage": fp16,
         "batched": batched,

@@ -6,7 +6,10 @@ Reads the rocpd SQLite database (or a *kernel_stats.csv) that
 `rocprofv3 --kernel-trace --stats` writes and prints calls, total and average
 duration and share per kernel, with the command that produced the trace.
 With --by-grid (database input only) each kernel is split by launch grid size,
-which separates the launches of one kernel on differently sized levels.
+which separates the launches of one kernel on differently sized levels. With
+--from KERNEL (database input only) only the dispatches from the first launch
+of a kernel whose name contains KERNEL on are counted (e.g. k_laplace_pcg: the
+solves, without the set-up).
 With --pmc the input is the *counter_collection.csv of a `rocprofv3 --pmc`
 run (written with --output-format csv): the mean value of each counter per
 kernel and grid size is printed.
@@ -34,15 +37,30 @@ def grid_column(con):
     raise SystemExit(f"no grid-size column in the kernels view (columns: {cols})")
 
 
-def from_db(path, by_grid=False):
+def start_column(con):
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)").fetchall()]
+    for c in cols:
+        if c.lower() == "start":
+            return c
+    raise SystemExit(f"no start column in the kernels view (columns: {cols})")
+
+
+def from_db(path, by_grid=False, first=None):
     con = sqlite3.connect(path)
+    where = ""
+    if first:
+        st = start_column(con)
+        t0 = con.execute(f"select min({st}) from kernels where name like ?", (f"%{first}%",)).fetchone()[0]
+        if t0 is None:
+            raise SystemExit(f"no dispatch of a kernel named like {first}")
+        where = f" where {st} >= {int(t0)}"
     if by_grid:
         g = grid_column(con)
-        q = f"select name, {g}, count(*), sum(duration), avg(duration) from kernels group by name, {g}"
+        q = f"select name, {g}, count(*), sum(duration), avg(duration) from kernels{where} group by name, {g}"
         rows = [(f"{short(n)} [grid {gx}]", c, t, a) for n, gx, c, t, a in con.execute(q).fetchall()]
     else:
         rows = [(short(n), c, t, a) for n, c, t, a in
-                con.execute("select name, count(*), sum(duration), avg(duration) from kernels group by name").fetchall()]
+                con.execute(f"select name, count(*), sum(duration), avg(duration) from kernels{where} group by name").fetchall()]
     con.close()
     return [(n, int(c), float(t) / 1e3, float(a) / 1e3) for n, c, t, a in rows]  # ns -> us
 
@@ -89,6 +107,7 @@ def main():
     ap.add_argument("--by-grid", action="store_true")
     ap.add_argument("--top", type=int, default=0, help="print only the N largest rows")
     ap.add_argument("--pmc", action="store_true")
+    ap.add_argument("--from", dest="first", default=None, help="count dispatches from the first launch of this kernel on")
     a = ap.parse_args()
     if a.pmc:
         text = "\n".join(main_pmc(a)) + "\n"
@@ -99,7 +118,9 @@ def main():
         return
     dbs = glob.glob(os.path.join(a.dir, "**", "*_results.db"), recursive=True)
     csvs = glob.glob(os.path.join(a.dir, "**", "*kernel_stats.csv"), recursive=True)
-    rows = from_db(dbs[0], a.by_grid) if dbs else from_csv(csvs[0])
+    if a.first and not dbs:
+        raise SystemExit("--from needs the rocpd database output")
+    rows = from_db(dbs[0], a.by_grid, a.first) if dbs else from_csv(csvs[0])
     agg = {}
     for n, c, t, _ in rows:
         c0, t0 = agg.get(n, (0, 0.0))
@@ -108,6 +129,8 @@ def main():
     lines = []
     if a.cmd:
         lines.append(f"command: {a.cmd}")
+    if a.first:
+        lines.append(f"dispatches from the first {a.first} on")
     lines.append(f"total kernel time: {total / 1e3:.1f} ms over {sum(c for c, _ in agg.values())} launches")
     lines.append(f"{'kernel':64s} {'calls':>9s} {'total ms':>10s} {'avg us':>9s} {'share':>7s}")
     ranked = sorted(agg.items(), key=lambda kv: -kv[1][1])
