@@ -428,6 +428,7 @@ __global__ void k_csr_mv(int rows, const int* __restrict__ p, const int* __restr
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
     float acc = 0.f;
+#pragma unroll 8
     for (int e = p[i]; e < p[i + 1]; ++e) acc += v[e] * x[ix[e]];
     y[i] = acc;
 }
@@ -439,9 +440,12 @@ __global__ void k_schur_rhs(int rows, const float* __restrict__ b, const int* __
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= rows) return;
     float acc = b[c];
+#pragma unroll 8
     for (int e = tp[c]; e < tp[c + 1]; ++e) acc += tv[e] * vx[ti[e]];
-    if (ap)
+    if (ap) {
+#pragma unroll 8
         for (int e = ap[c]; e < ap[c + 1]; ++e) acc += av[e] * (-u[ai[e]]);
+    }
     out[c] = acc;
 }
 
@@ -451,6 +455,7 @@ __global__ void k_prolong_add(int rows, const int* __restrict__ p, const int* __
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
     float acc = 0.f;
+#pragma unroll 8
     for (int e = p[i]; e < p[i + 1]; ++e) acc += v[e] * uc[ix[e]];
     u[i] += acc;
 }
