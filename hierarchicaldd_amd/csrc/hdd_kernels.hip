@@ -252,39 +252,54 @@ __global__ void k_copy(float* __restrict__ d, const float* __restrict__ s, size_
     for (; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
 }
 
-__device__ __forceinline__ int lut_page(const DevLevel& L, int x, int y, int z) {
-    const int bx = x >> 2, by = y >> 3, bz = z >> 3;
-    if (x < 0 || y < 0 || z < 0 || bx >= L.nbx || by >= L.nby || bz >= L.nbz) return L.np;
+// page of block (bx, by, bz) of a level (the zero page outside the block box or where none is allocated)
+__device__ __forceinline__ int lut_block(const DevLevel& L, int bx, int by, int bz) {
+    if (bx < 0 || by < 0 || bz < 0 || bx >= L.nbx || by >= L.nby || bz >= L.nbz) return L.np;
     const int q = L.lut[(bx * L.nby + by) * L.nbz + bz];
     return q < 0 ? L.np : q;
 }
 
-// coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only)
+// coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only).
+// The stencils of a coarse page (origin o) reach fine cells 2o-2 .. 2o+7 in x
+// and 2o-2 .. 2o+15 in y and z: 3 x 3 x 3 fine blocks, whose pages are looked
+// up once per workgroup.
 template <class S>
 __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const S* __restrict__ r_f, S* __restrict__ b_c) {
+    __shared__ int tab[27];
     const int p = xcd_page(C.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= C.np || page_stopped(C, p)) return;
+    if (p >= C.np || page_stopped(C, p)) return;  // uniform over the workgroup
+    const int ox = C.base[p * 3 + 0], oy = C.base[p * 3 + 1], oz = C.base[p * 3 + 2];
+    const int bx0 = (2 * ox - 2) >> 2, by0 = (2 * oy - 2) >> 3, bz0 = (2 * oz - 2) >> 3;
+    if (c < 27) tab[c] = lut_block(F, bx0 + c / 9, by0 + (c / 3) % 3, bz0 + c % 3);
+    __syncthreads();
     if (!(C.flags[e] & kActive)) return;
-    const int x = C.base[p * 3 + 0] + (c >> 6), y = C.base[p * 3 + 1] + ((c >> 3) & 7), z = C.base[p * 3 + 2] + (c & 7);
+    const int x = ox + (c >> 6), y = oy + ((c >> 3) & 7), z = oz + (c & 7);
     const int fx = 2 * x - 2, fy = 2 * y - 2, fz = 2 * z - 2;
     float acc = 0.f;
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j)
             for (int k = 0; k < 4; ++k) {
                 const int X = fx + i, Y = fy + j, Z = fz + k;
-                const int q = lut_page(F, X, Y, Z);
+                const int q = tab[((X >> 2) - bx0) * 9 + ((Y >> 3) - by0) * 3 + ((Z >> 3) - bz0)];
                 acc += c_rweights[i * 16 + j * 4 + k] * ld(r_f, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
             }
     st(b_c, e, acc);
 }
 
-// fine u += sum_{2x2x2} w u_coarse (Active fine cells only)
+// fine u += sum_{2x2x2} w u_coarse (Active fine cells only). The coarse cells
+// a fine page (origin o) reads lie in o/2 .. o/2+2 (x) and o/2 .. o/2+4 (y, z):
+// 2 x 2 x 2 coarse blocks, whose pages are looked up once per workgroup.
 template <class S>
 __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, S* __restrict__ u_f, const S* __restrict__ u_c) {
+    __shared__ int tab[8];
     const int p = xcd_page(F.np), c = threadIdx.x, e = p * 256 + c;
-    if (p >= F.np || page_stopped(F, p)) return;
+    if (p >= F.np || page_stopped(F, p)) return;  // uniform over the workgroup
+    const int ox = F.base[p * 3 + 0], oy = F.base[p * 3 + 1], oz = F.base[p * 3 + 2];
+    const int bx0 = (ox >> 1) >> 2, by0 = (oy >> 1) >> 3, bz0 = (oz >> 1) >> 3;
+    if (c < 8) tab[c] = lut_block(C, bx0 + (c >> 2), by0 + ((c >> 1) & 1), bz0 + (c & 1));
+    __syncthreads();
     if (!(F.flags[e] & kActive)) return;
-    const int x = F.base[p * 3 + 0] + (c >> 6), y = F.base[p * 3 + 1] + ((c >> 3) & 7), z = F.base[p * 3 + 2] + (c & 7);
+    const int x = ox + (c >> 6), y = oy + ((c >> 3) & 7), z = oz + (c & 7);
     const int cx = (x + 1) >> 1, cy = (y + 1) >> 1, cz = (z + 1) >> 1;
     const int px = 1 - (x - (2 * cx - 1)), py = 1 - (y - (2 * cy - 1)), pz = 1 - (z - (2 * cz - 1));
     float u = ld(u_f, e);
@@ -292,7 +307,7 @@ __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, S* __re
         for (int j = 0; j < 2; ++j)
             for (int k = 0; k < 2; ++k) {
                 const int X = cx + i - px, Y = cy + j - py, Z = cz + k - pz;
-                const int q = lut_page(C, X, Y, Z);
+                const int q = tab[((X >> 2) - bx0) * 4 + ((Y >> 3) - by0) * 2 + ((Z >> 3) - bz0)];
                 u += c_pweights[((i ^ px) << 2) | ((j ^ py) << 1) | (k ^ pz)] * ld(u_c, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
             }
     st(u_f, e, u);
