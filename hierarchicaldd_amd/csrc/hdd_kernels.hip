@@ -260,55 +260,73 @@ __device__ __forceinline__ int lut_block(const DevLevel& L, int bx, int by, int 
 }
 
 // coarse b = sum_{4x4x4} w r_fine over fine cells 2c-2 .. 2c+1 (Active coarse cells only).
-// The stencils of a coarse page (origin o) reach fine cells 2o-2 .. 2o+7 in x
-// and 2o-2 .. 2o+15 in y and z: 3 x 3 x 3 fine blocks, whose pages are looked
-// up once per workgroup.
+// The stencils of a coarse page (origin o) reach the fine region 2o-2 .. 2o+7
+// in x and 2o-2 .. 2o+15 in y and z (10 x 18 x 18 cells in 3 x 3 x 3 fine
+// blocks): its pages are looked up once and the region is staged in LDS
+// once, so each fine value is read from memory once instead of by up to 8
+// stencils; every stencil then sums the same values in the same order.
 template <class S>
 __global__ void __launch_bounds__(256) k_restrict(DevLevel F, DevLevel C, const S* __restrict__ r_f, S* __restrict__ b_c) {
+    constexpr int RX = 10, RY = 18, RZ = 18;
     __shared__ int tab[27];
+    __shared__ float sr[RX * RY * RZ];
     const int p = xcd_page(C.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= C.np || page_stopped(C, p)) return;  // uniform over the workgroup
+    const uint32_t fl = C.flags[e];  // the cell's own load goes out before the staging's
     const int ox = C.base[p * 3 + 0], oy = C.base[p * 3 + 1], oz = C.base[p * 3 + 2];
-    const int bx0 = (2 * ox - 2) >> 2, by0 = (2 * oy - 2) >> 3, bz0 = (2 * oz - 2) >> 3;
+    const int X0 = 2 * ox - 2, Y0 = 2 * oy - 2, Z0 = 2 * oz - 2;
+    const int bx0 = X0 >> 2, by0 = Y0 >> 3, bz0 = Z0 >> 3;
     if (c < 27) tab[c] = lut_block(F, bx0 + c / 9, by0 + (c / 3) % 3, bz0 + c % 3);
     __syncthreads();
-    if (!(C.flags[e] & kActive)) return;
-    const int x = ox + (c >> 6), y = oy + ((c >> 3) & 7), z = oz + (c & 7);
-    const int fx = 2 * x - 2, fy = 2 * y - 2, fz = 2 * z - 2;
+    for (int t = c; t < RX * RY * RZ; t += 256) {
+        const int X = X0 + t / (RY * RZ), Y = Y0 + (t / RZ) % RY, Z = Z0 + t % RZ;
+        const int q = tab[((X >> 2) - bx0) * 9 + ((Y >> 3) - by0) * 3 + ((Z >> 3) - bz0)];
+        sr[t] = ld(r_f, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
+    }
+    __syncthreads();
+    if (!(fl & kActive)) return;
+    const int lx = 2 * (c >> 6), ly = 2 * ((c >> 3) & 7), lz = 2 * (c & 7);  // stencil origin in the region
     float acc = 0.f;
     for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j)
-            for (int k = 0; k < 4; ++k) {
-                const int X = fx + i, Y = fy + j, Z = fz + k;
-                const int q = tab[((X >> 2) - bx0) * 9 + ((Y >> 3) - by0) * 3 + ((Z >> 3) - bz0)];
-                acc += c_rweights[i * 16 + j * 4 + k] * ld(r_f, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
-            }
+            for (int k = 0; k < 4; ++k)
+                acc += c_rweights[i * 16 + j * 4 + k] * sr[((lx + i) * RY + (ly + j)) * RZ + (lz + k)];
     st(b_c, e, acc);
 }
 
 // fine u += sum_{2x2x2} w u_coarse (Active fine cells only). The coarse cells
-// a fine page (origin o) reads lie in o/2 .. o/2+2 (x) and o/2 .. o/2+4 (y, z):
-// 2 x 2 x 2 coarse blocks, whose pages are looked up once per workgroup.
+// a fine page (origin o) reads form the region o/2 .. o/2+2 (x) and
+// o/2 .. o/2+4 (y, z), 3 x 5 x 5 cells in 2 x 2 x 2 coarse blocks: its pages
+// are looked up once and the region is staged in LDS once per workgroup.
 template <class S>
 __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, S* __restrict__ u_f, const S* __restrict__ u_c) {
+    constexpr int RX = 3, RY = 5, RZ = 5;
     __shared__ int tab[8];
+    __shared__ float su[RX * RY * RZ];
     const int p = xcd_page(F.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= F.np || page_stopped(F, p)) return;  // uniform over the workgroup
+    const uint32_t fl = F.flags[e];  // the cell's own loads go out before the staging's
+    float u = ld(u_f, e);
     const int ox = F.base[p * 3 + 0], oy = F.base[p * 3 + 1], oz = F.base[p * 3 + 2];
-    const int bx0 = (ox >> 1) >> 2, by0 = (oy >> 1) >> 3, bz0 = (oz >> 1) >> 3;
+    const int X0 = ox >> 1, Y0 = oy >> 1, Z0 = oz >> 1;
+    const int bx0 = X0 >> 2, by0 = Y0 >> 3, bz0 = Z0 >> 3;
     if (c < 8) tab[c] = lut_block(C, bx0 + (c >> 2), by0 + ((c >> 1) & 1), bz0 + (c & 1));
     __syncthreads();
-    if (!(F.flags[e] & kActive)) return;
+    if (c < RX * RY * RZ) {
+        const int X = X0 + c / (RY * RZ), Y = Y0 + (c / RZ) % RY, Z = Z0 + c % RZ;
+        const int q = tab[((X >> 2) - bx0) * 4 + ((Y >> 3) - by0) * 2 + ((Z >> 3) - bz0)];
+        su[c] = ld(u_c, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
+    }
+    __syncthreads();
+    if (!(fl & kActive)) return;
     const int x = ox + (c >> 6), y = oy + ((c >> 3) & 7), z = oz + (c & 7);
     const int cx = (x + 1) >> 1, cy = (y + 1) >> 1, cz = (z + 1) >> 1;
     const int px = 1 - (x - (2 * cx - 1)), py = 1 - (y - (2 * cy - 1)), pz = 1 - (z - (2 * cz - 1));
-    float u = ld(u_f, e);
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int k = 0; k < 2; ++k) {
                 const int X = cx + i - px, Y = cy + j - py, Z = cz + k - pz;
-                const int q = tab[((X >> 2) - bx0) * 4 + ((Y >> 3) - by0) * 2 + ((Z >> 3) - bz0)];
-                u += c_pweights[((i ^ px) << 2) | ((j ^ py) << 1) | (k ^ pz)] * ld(u_c, q * 256 + ((X & 3) << 6 | (Y & 7) << 3 | (Z & 7)));
+                u += c_pweights[((i ^ px) << 2) | ((j ^ py) << 1) | (k ^ pz)] * su[((X - X0) * RY + (Y - Y0)) * RZ + (Z - Z0)];
             }
     st(u_f, e, u);
 }
