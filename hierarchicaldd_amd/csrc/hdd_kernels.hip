@@ -175,11 +175,17 @@ __global__ void __launch_bounds__(256) k_smooth_local(DevLevel L, const int* __r
     float zmc = 0.f, zpc = 0.f;
     uint32_t f[4] = {0u, 0u, 0u, 0u};
     int sl[6] = {-1, -1, -1, -1, -1, -1};
+    float wgt[4] = {0.f, 0.f, 0.f, 0.f};  // omega / face count of each cell, 0 where no sweep writes it
     if (live) {
         const int* nb = L.nbr + p * 6;
         const size_t base = (size_t)p * 64;
         const uint4 fv = reinterpret_cast<const uint4*>(L.flags)[base + l];
         f[0] = fv.x; f[1] = fv.y; f[2] = fv.z; f[3] = fv.w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t sel = f[k] & (kActive | kBoundary);
+            if (sel == kActive || sel == (kActive | kBoundary)) wgt[k] = c_omega_fc[__popc(f[k] & (0x3fu * kFaceMX))];
+        }
 #pragma unroll
         for (int q = 0; q < 6; ++q) sl[q] = nb_slot[p * 6 + q];
         st4(us[0][w], l, ld4(u_in, base + l));
@@ -215,7 +221,7 @@ __global__ void __launch_bounds__(256) k_smooth_local(DevLevel L, const int* __r
                 const float r = bv.v[k] - result;
                 const uint32_t sel = f[k] & (kActive | kBoundary);
                 float u = cv;
-                if (sel == kActive || sel == (kActive | kBoundary)) u += c_omega_fc[__popc(f[k] & (0x3fu * kFaceMX))] * r;
+                if (sel == kActive || sel == (kActive | kBoundary)) u += wgt[k] * r;
                 out.v[k] = u;
             }
             st4(us[1 - cur][w], l, out);
