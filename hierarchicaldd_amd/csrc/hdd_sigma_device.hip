@@ -652,8 +652,11 @@ static void dense_top_inverse(const TriPlanHost::TopBlock& B, float* P) {
     }
     dI.alloc((size_t)m * m);
     static std::mutex mu;
-    static rocblas_handle handle = nullptr;
+    static std::map<int, rocblas_handle> handles;  // one per device: a handle belongs to the device current at its creation
     std::lock_guard<std::mutex> lock(mu);
+    int dev = 0;
+    hdd_check(hipGetDevice(&dev), "hipGetDevice");
+    rocblas_handle& handle = handles[dev];
     if (!handle) rocblas_check(rocblas_create_handle(&handle), "create handle");
     rocblas_check(rocblas_set_stream(handle, nullptr), "set stream");
     rocblas_check(rocblas_dtrtri(handle, rocblas_fill_lower, rocblas_diagonal_non_unit, m, dL.p, m, dI.p, m), "dtrtri");
