@@ -17,7 +17,7 @@ def box32(oracle_lib):
     return dom, cfg, oracle_lib.Oracle(dom, cfg)
 
 
-def test_config0_counts_match_independent_survey():
+def test_config0_counts_match_independent_count():
     """The 64^3 config: 1,003 pages, 221,251 Active cells (counted independently of this code)."""
     dom, cfg = domain.config0()
     assert dom.n_pages == 1003
