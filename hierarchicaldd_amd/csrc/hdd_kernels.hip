@@ -153,83 +153,68 @@ __global__ void __launch_bounds__(256) k_smooth(DevLevel L, const S* __restrict_
 // All `count` sweeps (interior and boundary cells) of a level whose Active
 // cells couple only within small clusters of pages (the bottom of the V-cycle:
 // each subdomain's coarsest block lies in one or two pages). A workgroup holds
-// up to four pages, whole clusters (wg_page, -1 for an empty slot), one
-// wavefront per page, and sweeps them in LDS with the operations of Stencil4
-// and k_smooth, storing every iterate in the level's storage type, so the
-// result is bit-identical to `count` launches of k_smooth. A face neighbour
-// page of the same cluster is read from its LDS slot (nb_slot[p*6+f], -1:
-// none); other neighbour pages hold no cell a coupled Active cell reads that a
-// sweep writes, so they are read once. Every wavefront takes part in the
-// barriers; only those with a live page compute. The result goes to u_out, a
-// buffer other than u_in (other workgroups read u_in's pages).
+// up to four pages, whole clusters (wg_page, -1 for an empty slot), 256
+// threads per page, thread t owning cell t, and sweeps them in LDS with the
+// operations of Stencil4 and k_smooth, storing every iterate in the level's
+// storage type, so the result is bit-identical to `count` launches of
+// k_smooth. A face neighbour page of the same cluster is read from its LDS
+// slot (nb_slot[p*6+f], -1: none); other neighbour pages hold no cell a
+// coupled Active cell reads that a sweep writes, so they are read once. Every
+// thread takes part in the barriers; only those of a live page compute. The
+// result goes to u_out, a buffer other than u_in (other workgroups read u_in's
+// pages).
 template <class S>
-__global__ void __launch_bounds__(256) k_smooth_local(DevLevel L, const int* __restrict__ wg_page, const int* __restrict__ nb_slot,
-                                                      const S* __restrict__ u_in, S* __restrict__ u_out, const S* __restrict__ b,
-                                                      int count) {
-    __shared__ __align__(16) S us[2][4][kPageCells];
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+__global__ void __launch_bounds__(1024) k_smooth_local(DevLevel L, const int* __restrict__ wg_page, const int* __restrict__ nb_slot,
+                                                       const S* __restrict__ u_in, S* __restrict__ u_out, const S* __restrict__ b,
+                                                       int count) {
+    __shared__ S us[2][4][kPageCells];
+    const int c = threadIdx.x & 255, w = threadIdx.x >> 8;
     const int p = wg_page[blockIdx.x * 4 + w];
     const bool live = p >= 0 && !page_stopped(L, p);
-    const int x = l >> 4, y = (l >> 1) & 7, zh = l & 1;
-    V4 xmc = {{0.f, 0.f, 0.f, 0.f}}, xpc = xmc, ymc = xmc, ypc = xmc, bv = xmc;
-    float zmc = 0.f, zpc = 0.f;
-    uint32_t f[4] = {0u, 0u, 0u, 0u};
-    int sl[6] = {-1, -1, -1, -1, -1, -1};
-    float wgt[4] = {0.f, 0.f, 0.f, 0.f};  // omega / face count of each cell, 0 where no sweep writes it
+    const int x = c >> 6, y = (c >> 3) & 7, z = c & 7;
+    // per face: the in-page neighbour cell, or (at the page face) the
+    // neighbour page's cell, read from its LDS slot or once from u_in
+    const int nc[6] = {x > 0 ? c - 64 : c + 192, x < 3 ? c + 64 : c - 192, y > 0 ? c - 8 : c + 56,
+                       y < 7 ? c + 8 : c - 56, z > 0 ? c - 1 : c + 7, z < 7 ? c + 1 : c - 7};
+    const bool inside[6] = {x > 0, x < 3, y > 0, y < 7, z > 0, z < 7};
+    int slot[6];
+    float cst[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t f = 0u;
+    float bv = 0.f, wgt = 0.f;
     if (live) {
-        const int* nb = L.nbr + p * 6;
-        const size_t base = (size_t)p * 64;
-        const uint4 fv = reinterpret_cast<const uint4*>(L.flags)[base + l];
-        f[0] = fv.x; f[1] = fv.y; f[2] = fv.z; f[3] = fv.w;
+        f = L.flags[(size_t)p * 256 + c];
+        const uint32_t sel = f & (kActive | kBoundary);
+        if (sel == kActive || sel == (kActive | kBoundary)) wgt = c_omega_fc[__popc(f & (0x3fu * kFaceMX))];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t sel = f[k] & (kActive | kBoundary);
-            if (sel == kActive || sel == (kActive | kBoundary)) wgt[k] = c_omega_fc[__popc(f[k] & (0x3fu * kFaceMX))];
+        for (int q = 0; q < 6; ++q) {
+            slot[q] = inside[q] ? w : nb_slot[p * 6 + q];
+            if (slot[q] < 0) cst[q] = ld(u_in, (size_t)L.nbr[p * 6 + q] * 256 + nc[q]);
         }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) sl[q] = nb_slot[p * 6 + q];
-        st4(us[0][w], l, ld4(u_in, base + l));
-        if (x == 0 && sl[0] < 0) xmc = ld4(u_in, (size_t)nb[0] * 64 + l + 48);
-        if (x == 3 && sl[1] < 0) xpc = ld4(u_in, (size_t)nb[1] * 64 + l - 48);
-        if (y == 0 && sl[2] < 0) ymc = ld4(u_in, (size_t)nb[2] * 64 + l + 14);
-        if (y == 7 && sl[3] < 0) ypc = ld4(u_in, (size_t)nb[3] * 64 + l - 14);
-        if (!zh && sl[4] < 0) zmc = ld(u_in, (size_t)nb[4] * 256 + 4 * l + 7);
-        if (zh && sl[5] < 0) zpc = ld(u_in, (size_t)nb[5] * 256 + 4 * l - 4);
-        bv = ld4(b, base + l);
+        bv = ld(b, (size_t)p * 256 + c);
+        us[0][w][c] = u_in[(size_t)p * 256 + c];
     }
     __syncthreads();
     int cur = 0;
     for (int it = 0; it < count; ++it) {
         if (live) {
-            const S* uu = us[cur][w];
-            const V4 uc = ld4(uu, l);
-            const V4 xm = x > 0 ? ld4(uu, l - 16) : sl[0] >= 0 ? ld4(us[cur][sl[0]], l + 48) : xmc;
-            const V4 xp = x < 3 ? ld4(uu, l + 16) : sl[1] >= 0 ? ld4(us[cur][sl[1]], l - 48) : xpc;
-            const V4 ym = y > 0 ? ld4(uu, l - 2) : sl[2] >= 0 ? ld4(us[cur][sl[2]], l + 14) : ymc;
-            const V4 yp = y < 7 ? ld4(uu, l + 2) : sl[3] >= 0 ? ld4(us[cur][sl[3]], l - 14) : ypc;
-            const float zm = zh ? ld(uu, 4 * l - 1) : sl[4] >= 0 ? ld(us[cur][sl[4]], 4 * l + 7) : zmc;
-            const float zp = zh ? (sl[5] >= 0 ? ld(us[cur][sl[5]], 4 * l - 4) : zpc) : ld(uu, 4 * l + 4);
-            V4 out;
+            const float cv = ld(us[cur][w], c);
+            float nv[6];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float cv = uc.v[k];
-                const float nv[6] = {xm.v[k], xp.v[k], ym.v[k], yp.v[k], k > 0 ? uc.v[k - 1] : zm, k < 3 ? uc.v[k + 1] : zp};
-                float result = 0.f;
+            for (int q = 0; q < 6; ++q) nv[q] = slot[q] >= 0 ? ld(us[cur][slot[q]], nc[q]) : cst[q];
+            float result = 0.f;
 #pragma unroll
-                for (int q = 0; q < 6; ++q)
-                    if (f[k] & (kFaceMX << q)) result -= (nv[q] - cv);
-                const float r = bv.v[k] - result;
-                const uint32_t sel = f[k] & (kActive | kBoundary);
-                float u = cv;
-                if (sel == kActive || sel == (kActive | kBoundary)) u += wgt[k] * r;
-                out.v[k] = u;
-            }
-            st4(us[1 - cur][w], l, out);
+            for (int q = 0; q < 6; ++q)
+                if (f & (kFaceMX << q)) result -= (nv[q] - cv);
+            const float r = bv - result;
+            const uint32_t sel = f & (kActive | kBoundary);
+            float u = cv;
+            if (sel == kActive || sel == (kActive | kBoundary)) u += wgt * r;
+            st(us[1 - cur][w], c, u);
         }
         __syncthreads();
         cur = 1 - cur;
     }
-    if (live) st4(u_out, (size_t)p * 64 + l, ld4(us[cur][w], l));
+    if (live) u_out[(size_t)p * 256 + c] = us[cur][w][c];
 }
 
 // residual at Active cells; other cells of r are written 0 (the value they
@@ -529,7 +514,7 @@ static void smooth_t(const DevLevel& L, const S* u_in, S* u_out, const S* b, boo
 template <class S>
 static void smooth_local_t(const DevLevel& L, const int* wg_page, int nwg, const int* nb_slot, const S* u_in, S* u_out, const S* b,
                            int count, hipStream_t s) {
-    if (nwg) hipLaunchKernelGGL(k_smooth_local<S>, dim3(nwg), dim3(256), 0, s, L, wg_page, nb_slot, u_in, u_out, b, count);
+    if (nwg) hipLaunchKernelGGL(k_smooth_local<S>, dim3(nwg), dim3(1024), 0, s, L, wg_page, nb_slot, u_in, u_out, b, count);
 }
 template <class S>
 static void residual_t(const DevLevel& L, const S* u, const S* b, S* r, hipStream_t s) {
