@@ -186,9 +186,9 @@ def main():
 
 
 def sweep_grid_threads(pages):
-    """Launch grid (threads) of the level-0 sweep: four pages per 256-thread workgroup, XCD-chunked."""
-    groups = (pages + 3) // 4
-    return 8 * ((groups + 7) // 8) * 256
+    """Launch grid (threads) of the level-0 sweep: eight pages per 512-thread workgroup, XCD-chunked."""
+    groups = (pages + 7) // 8
+    return 8 * ((groups + 7) // 8) * 512
 
 
 def counter_traffic(pages, kernel="k_smooth<float>"):

@@ -65,7 +65,7 @@ __device__ __forceinline__ void st(__half* __restrict__ p, size_t i, float v) { 
 // neighbours inside the page are the vectors of lanes l+-16 (x) and l+-2 (y)
 // and the neighbouring elements (z); across a page face they come from the
 // page's 6-entry neighbour table (the zero page `np` stands in for unallocated
-// pages). Four pages per 256-thread workgroup.
+// pages). Four pages per 256-thread workgroup (the sweep: eight per 512).
 struct V4 { float v[4]; };
 __device__ __forceinline__ V4 ld4(const float* __restrict__ p, size_t i4) {
     const float4 t = reinterpret_cast<const float4*>(p)[i4];
@@ -127,15 +127,18 @@ struct Stencil4 {
 };
 
 __host__ __device__ inline int page_groups(int np) { return (np + 3) >> 2; }
+// the sweep takes eight pages per 512-thread workgroup (measured 1.7% faster than four at 256^3)
+constexpr int kSweepPages = 8;
+__host__ __device__ inline int sweep_groups(int np) { return (np + kSweepPages - 1) / kSweepPages; }
 
 // One damped-Jacobi sweep (residual + correction fused, ping-pong buffers);
 // u += (omega / face count) r on the selected cells
 // (LIB/SPGrid_Correction_Legacy.h:60-77), other cells copied unchanged.
 template <class S>
-__global__ void __launch_bounds__(256) k_smooth(DevLevel L, const S* __restrict__ u_in, S* __restrict__ u_out,
-                                                const S* __restrict__ b, int write_interior, int write_boundary) {
-    const int ng = page_groups(L.np), gi = xcd_page(ng), l = threadIdx.x & 63;
-    const int p = gi * 4 + (int)(threadIdx.x >> 6);
+__global__ void __launch_bounds__(64 * kSweepPages) k_smooth(DevLevel L, const S* __restrict__ u_in, S* __restrict__ u_out,
+                                                             const S* __restrict__ b, int write_interior, int write_boundary) {
+    const int ng = sweep_groups(L.np), gi = xcd_page(ng), l = threadIdx.x & 63;
+    const int p = gi * kSweepPages + (int)(threadIdx.x >> 6);
     if (gi >= ng || p >= L.np || page_stopped(L, p)) return;
     const Stencil4<S> st(L, p, l, u_in, b);
     V4 out;
@@ -508,7 +511,7 @@ void upload_constants() {
 
 template <class S>
 static void smooth_t(const DevLevel& L, const S* u_in, S* u_out, const S* b, bool interior, bool boundary, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_smooth<S>, dim3(kXcds * xcd_chunk(page_groups(L.np))), dim3(256), 0, s, L, u_in, u_out, b, (int)interior,
+    if (L.np) hipLaunchKernelGGL(k_smooth<S>, dim3(kXcds * xcd_chunk(sweep_groups(L.np))), dim3(64 * kSweepPages), 0, s, L, u_in, u_out, b, (int)interior,
                                  (int)boundary);
 }
 template <class S>
