@@ -70,6 +70,9 @@ def main():
                     help="largest request edge of the served leg (384 for the full stream; 256 keeps the default run short)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="set up, then run only level-0 sweeps and a plain copy of known size (for counter runs)")
+    ap.add_argument("--plan-only", default=None, metavar="LEVEL,PLAN",
+                    help="set up, then run 10 plain copies of known size and 1 + 10 solves of one triangular-solve "
+                         "plan (PLAN: Aii, Arr or bottom), for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -90,6 +93,14 @@ def main():
     log(f"[bench] setup {time.time() - t0:.1f}s (factorisation {st['factor_s']:.1f}s, interface-solver plans "
         f"{st['sigma_device_bytes'] / 2**30:.2f} GiB, bottom rho {st['bottom_rho']:.4f}, direct bottom {bool(st['bottom_direct'])})")
     tol = 1e-6 * float(np.abs(dom.b).max())
+
+    if args.plan_only:
+        lvl, plan = args.plan_only.split(",")
+        copy_ms, copy_bytes = solver.profile_copy(10)
+        ms, by = solver.profile_solve(int(lvl), plan, 10)
+        log(f"[bench] calibration copy: {copy_bytes / 2:.0f} bytes read and {copy_bytes / 2:.0f} written per launch")
+        log(f"[bench] level {lvl} {plan} solve: {ms * 1e3:.1f} us, {by / 1e6:.1f} MB algorithmic, 11 solves after the copies")
+        return
 
     if args.sweep_only:
         sweep_ms, sweep_bytes = solver.profile_sweep(20)
@@ -114,23 +125,32 @@ def main():
     wall = time.perf_counter() - t_start
     ms_per_step = 1000.0 * wall / args.steps
 
-    # roofline of the level-0 smoothing sweep, HIP events on the context stream
+    # the level-0 smoothing sweep (the V-cycle kernel), HIP events on the context stream
     sweep_ms, sweep_bytes = solver.profile_sweep(100)
-    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9
-    log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic, {achieved:.0f} GB/s")
+    sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    log(f"[bench] level-0 sweep: {sweep_ms * 1e3:.1f} us/launch, {sweep_bytes / 1e6:.1f} MB algorithmic, {sweep_gbs:.0f} GB/s")
+    copy_ms, copy_bytes = solver.profile_copy(50)
+    copy_gbs = copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"[bench] plain copy of one level-0 channel: {copy_ms * 1e3:.1f} us, {copy_gbs:.0f} GB/s (read + write)")
 
-    # exact solves of the interface solver's triangular-solve plans (the kernels that take most of a
-    # PCG iteration): mean time per solve with HIP events, algorithmic bytes = 4 B per factor entry
-    # per pass, two passes (subdomains that share one factor counted each)
+    # exact solves of the interface solver's triangular-solve plans: mean time per solve with HIP
+    # events, algorithmic bytes = 4 B per factor entry per pass, two passes (subdomains that share
+    # one factor counted each); solves per PCG iteration from the sigma V-cycle's schedule
     sigma_solves = []
     n_sigma, _ = solver.sigma_info()
     plans = [(l, w) for l in range(n_sigma) for w in ("Aii", "Arr")] + ([(n_sigma - 1, "bottom")] if st["bottom_direct"] else [])
+    ms_per_it = 1000.0 * float(np.mean(times)) / max(1, int(np.median(iters)))
     for l, w in plans:
         ms, by = solver.profile_solve(l, w, 20)
         gbs = by / (ms * 1e-3) / 1e9
+        per_it = solves_per_iteration(l, w, n_sigma, cfg, bool(st["bottom_direct"]))
         sigma_solves.append({"level": l, "plan": w, "us_per_solve": ms * 1e3, "algorithmic_MB": by / 1e6,
-                             "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
-        log(f"[bench] sigma level {l} {w} solve: {ms * 1e3:.0f} us, {by / 1e6:.0f} MB algorithmic, {gbs:.0f} GB/s")
+                             "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS, "solves_per_iteration": per_it,
+                             "share_of_iteration": per_it * ms / ms_per_it})
+        log(f"[bench] sigma level {l} {w} solve: {ms * 1e3:.0f} us, {by / 1e6:.0f} MB algorithmic, {gbs:.0f} GB/s, "
+            f"{per_it} per iteration ({100 * per_it * ms / ms_per_it:.1f}% of it)")
+    dom_plan = max(sigma_solves, key=lambda r: r["share_of_iteration"])
+    dofs = dof_updates_per_iteration(solver, cfg)
 
     # FP16-storage V-cycles (FP32 arithmetic) on the same problem
     solver.set_fp16(True)
@@ -172,10 +192,20 @@ def main():
         "config": {"workload": dom.meta.get("workload"), "cells": "256^3", "pages": dom.n_pages,
                    "sub_size": cfg.sub_size, "vlevels": cfg.vlevels, "sigma_levels": cfg.sigma_levels,
                    "tolerance": "max|r| <= 1e-6 max|b|", "iterations": int(np.median(iters)),
-                   "time_to_tolerance_s": float(np.mean(times)), "setup_s": st["setup_s"]},
-        "roofline": {"kernel": "k_smooth (level-0 damped-Jacobi sweep)", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": counter_traffic(dom.n_pages) if args.config == "sphere256" else None},
+                   "time_to_tolerance_s": float(np.mean(times)), "setup_s": st["setup_s"], "factor_s": st["factor_s"]},
+        "roofline": {"kernel": f"level-{dom_plan['level']} {dom_plan['plan']} triangular-solve plan (k_tri_* / k_top_* chain, "
+                               f"{100 * dom_plan['share_of_iteration']:.0f}% of a PCG iteration)",
+                     "bound": "hbm", "achieved": dom_plan["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom_plan["achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": plan_counter_traffic(dom_plan["level"], dom_plan["plan"]) if args.config == "sphere256" else None,
+                     "algorithmic_bytes": dom_plan["algorithmic_MB"] * 1e6, "us_per_solve": dom_plan["us_per_solve"]},
+        "vcycle_sweep": {"kernel": "k_smooth (level-0 damped-Jacobi sweep)", "achieved": sweep_gbs, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "us_per_launch": sweep_ms * 1e3,
+                         "algorithmic_bytes": sweep_bytes,
+                         "traffic": counter_traffic(dom.n_pages) if args.config == "sphere256" else None},
+        "stream_copy_GBs": copy_gbs,
+        "dof_updates_per_s": dofs * int(np.median(iters)) / float(np.mean(times)),
+        "dof_updates_per_iteration": dofs,
         "sigma_solves": sigma_solves,
         "cpu_baseline": cpu,
         "fp16_storage": fp16,
@@ -183,6 +213,60 @@ def main():
         "served": served,
     }
     print(json.dumps(out), flush=True)
+
+
+def solves_per_iteration(level, plan, n_sigma, cfg, bottom_direct):
+    """Exact solves of one plan per PCG iteration (one preconditioner apply, sigma_mg V-cycles):
+    above the bottom, sigma_smoothing smooths down and up (each one Aii and one Arr solve) and one
+    residual (one Aii solve); the bottom's 200 smooths are one combined solve when it is direct."""
+    s, mg = cfg.sigma_smoothing, cfg.sigma_mg
+    if plan == "bottom":
+        return mg if bottom_direct else 0
+    if level < n_sigma - 1:
+        return mg * (2 * s + 1 if plan == "Aii" else 2 * s)
+    return 0 if bottom_direct else mg * 200
+
+
+def dof_updates_per_iteration(solver, cfg):
+    """Damped-Jacobi corrections of single Active cells per PCG iteration (one preconditioner apply =
+    2 * nVcycles V-cycles): on each level above the bottom, boundary_smoothing sweeps of the
+    Active&Boundary cells before and after interior_smoothing sweeps of the other Active cells, going
+    down and coming up; 200 sweeps of every Active cell at the bottom (LIB/SPGrid_V_Cycle_Helper.h:156-192)."""
+    from hierarchicaldd_amd import domain
+    per_vcycle = 0
+    for l in range(cfg.vlevels):
+        fl = solver.level(l)[1]
+        act = (fl & domain.ACTIVE) != 0
+        bnd = int((act & ((fl & domain.BOUNDARY) != 0)).sum())
+        inner = int(act.sum()) - bnd
+        if l < cfg.vlevels - 1:
+            per_vcycle += 2 * (2 * cfg.boundary_smoothing * bnd + cfg.interior_smoothing * inner)
+        else:
+            per_vcycle += 200 * int(act.sum())
+    return 2 * cfg.nVcycles * per_vcycle
+
+
+def plan_counter_traffic(level, plan):
+    """HBM bytes of one solve of a triangular-solve plan from the committed rocprofv3 counter
+    summaries (profiles/tri_plan_sphere256_l<level>_<plan>_{fetch,write}_size.txt): FETCH_SIZE and
+    WRITE_SIZE summed over every kernel dispatch of the measured solves, divided by the solves, each
+    scaled by what the counter reads for the calibration copy of the same run. None if absent."""
+    base = os.path.join(PROFILES, f"tri_plan_sphere256_l{level}_{plan.lower()}")
+    vals = {}
+    for counter, suffix in (("FETCH_SIZE", "fetch_size"), ("WRITE_SIZE", "write_size")):
+        path = f"{base}_{suffix}.txt"
+        if not os.path.exists(path):
+            return None
+        for line in open(path):
+            if line.startswith("per-solve KiB") or line.startswith("copy KiB") or line.startswith("copy bytes"):
+                k, v = line.split(":")
+                vals[(counter, k.strip())] = float(v)
+    try:
+        f = vals[("FETCH_SIZE", "per-solve KiB")] * vals[("FETCH_SIZE", "copy bytes")] / (1024.0 * vals[("FETCH_SIZE", "copy KiB")])
+        w = vals[("WRITE_SIZE", "per-solve KiB")] * vals[("WRITE_SIZE", "copy bytes")] / (1024.0 * vals[("WRITE_SIZE", "copy KiB")])
+    except KeyError:
+        return None
+    return 1024.0 * (f + w)
 
 
 def sweep_grid_threads(pages):
@@ -248,7 +332,7 @@ def sparse_leg(hdd):
     """The 512^3 union-of-spheres domain: FP32 and FP16-storage V-cycles at 1e-6 and 1e-8."""
     from hierarchicaldd_amd import domain
     t0 = time.time()
-    dom, cfg = domain.sparse_spheres()
+    dom, cfg = domain.sparse_spheres(target_coverage=0.30)
     meta = {k: v for k, v in dom.meta.items() if k != "spheres"}
     log(f"[sparse] {meta} pages={dom.n_pages} generated in {time.time() - t0:.1f}s")
     t0 = time.time()
@@ -266,17 +350,21 @@ def sparse_leg(hdd):
     for fp16 in (False, True):
         g.set_fp16(fp16)
         for tol in (1e-6, 1e-8):
-            it, hist, secs = g.solve(tol * bmax, 2000)
-            res = true_residual(dom, g.get(hdd.CH_X))
-            row = {"storage": "fp16" if fp16 else "fp32", "tolerance": tol, "iterations": it, "seconds": secs,
-                   "final_max_r_rel": float(hist[-1] / bmax), "true_residual_rel_fp64": res}
-            log(f"[sparse] {row}")
-            rows.append(row)
+            for outer in ("fp32", "fp64"):
+                g.set_outer_fp64(outer == "fp64")
+                it, hist, secs = g.solve(tol * bmax, 2000)
+                x = g.x_f64() if outer == "fp64" else g.get(hdd.CH_X)
+                res = true_residual(dom, x)
+                row = {"storage": "fp16" if fp16 else "fp32", "outer": outer, "tolerance": tol, "iterations": it, "seconds": secs,
+                       "final_max_r_rel": float(hist[-1] / bmax), "true_residual_rel_fp64": res}
+                log(f"[sparse] {row}")
+                rows.append(row)
+    g.set_outer_fp64(False)
     ms, by = g.profile_sweep(50)
     g.set_fp16(False)
     ms32, by32 = g.profile_sweep(50)
     g.close()
-    return {"metric": METRIC, "config": {"workload": "sparse512_spheres24", **meta, "pages": dom.n_pages, "sub_size": cfg.sub_size,
+    return {"metric": METRIC, "config": {**meta, "pages": dom.n_pages, "sub_size": cfg.sub_size,
                                          "vlevels": cfg.vlevels}, "setup_s": st["setup_s"], "solves": rows,
             "sweep_fp32": {"us": ms32 * 1e3, "GBs": by32 / (ms32 * 1e-3) / 1e9}, "sweep_fp16": {"us": ms * 1e3, "GBs": by / (ms * 1e-3) / 1e9}}
 
@@ -368,30 +456,66 @@ def served_leg(hdd, n_req, rates, nmax, seed=1):
     return out
 
 
+def cpu_info():
+    """CPU model, sockets and physical cores from lscpu (None where lscpu is absent)."""
+    import subprocess
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    info["model"] = kv.get("Model name")
+    try:
+        info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (KeyError, ValueError):
+        pass
+    return info
+
+
 def cpu_baseline(hdd):
-    """Oracle dd_pcg on the complete 64^3 problem (setup + solve), beside the GPU on the same problem."""
+    """The oracle (the CPU restatement of the reference dd_pcg) on a bounded sample with the
+    256^3 workload's DD parameters (sub_size 32, vlevels 4) on the 64^3 sphere domain, with the
+    host's OpenMP threads and with one thread, beside the GPU on the same sample."""
     from hierarchicaldd_amd import domain
     from oracle import pyoracle
     pyoracle.build()
-    dom, cfg = domain.config0()
+    dom, cfg = domain.sphere_case(64, 32, 4)
     tol = 1e-6 * float(np.abs(dom.b).max())
-    cores = os.cpu_count() or 1
-    t0 = time.perf_counter()
-    o = pyoracle.Oracle(dom, cfg)
-    it, hist, secs = o.solve(tol, 500)
-    total = time.perf_counter() - t0
+    info = cpu_info()
+    threads = pyoracle.lib().ora_max_threads()
+    runs = {}
+    for nt in (threads, 1):
+        prev = pyoracle.set_threads(nt)
+        try:
+            t0 = time.perf_counter()
+            o = pyoracle.Oracle(dom, cfg)
+            setup = time.perf_counter() - t0
+            it, hist, secs = o.solve(tol, 500)
+            del o
+        finally:
+            pyoracle.set_threads(prev)
+        runs[nt] = {"threads": nt, "setup_s": setup, "solve_s": secs, "iterations": it}
+        log(f"[bench] cpu baseline ({nt} threads): setup {setup:.1f}s, solve {secs:.2f}s ({it} iterations)")
     g = hdd.DDSolver(dom, cfg)
     g.solve(tol, 500)
     git, gh, gsecs = g.solve(tol, 500)
+    g_setup = g.setup_seconds
     g.close()
-    log(f"[bench] cpu baseline: oracle 64^3 setup {o.setup_seconds:.1f}s solve {secs:.2f}s ({it} iterations); "
-        f"gpu 64^3 solve {gsecs:.3f}s ({git} iterations)")
-    return {"value": 1.0 / secs, "unit": "solves/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-            "kind": "port", "sample": f"oracle dd_pcg on the 64^3 sphere problem (sub 16, vlevels 3): "
-                                      f"{it} iterations in {secs:.2f}s after {o.setup_seconds:.1f}s setup "
-                                      f"({total:.1f}s total); the GPU solves the same problem in {gsecs:.3f}s "
-                                      f"({git} iterations)",
-            "gpu_same_sample_solves_per_s": 1.0 / gsecs}
+    full = runs[threads]
+    log(f"[bench] gpu on the same sample: solve {gsecs:.4f}s ({git} iterations), setup {g_setup:.1f}s")
+    return {"value": 1.0 / full["solve_s"], "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"oracle dd_pcg (CPU restatement of the reference) on the 64^3 sphere domain with the 256^3 "
+                      f"workload's DD parameters (sub_size 32, vlevels 4, 8 subdomains): {full['iterations']} iterations, "
+                      f"solve {full['solve_s']:.2f}s after {full['setup_s']:.1f}s setup on {threads} threads; one thread: "
+                      f"solve {runs[1]['solve_s']:.2f}s, setup {runs[1]['setup_s']:.1f}s; the GPU solves the same problem "
+                      f"in {gsecs:.4f}s ({git} iterations)",
+            "cpu_model": info["model"], "physical_cores": info["physical_cores"], "logical_cpus": info["logical_cpus"],
+            "runs": list(runs.values()), "gpu_same_sample_solves_per_s": 1.0 / gsecs, "gpu_same_sample_setup_s": g_setup}
 
 
 if __name__ == "__main__":

@@ -38,19 +38,37 @@ def _compile(src):
     return obj
 
 
+KRYLOV_CHECK_SRC = os.path.join(HERE, "..", "tests", "krylov_adapter_check.cpp")
+KRYLOV_CHECK_LIB = os.path.join(OUT, "libhdd_krylov_check.so")
+
+
+def build_krylov_check() -> str:
+    """Compiles the test driver of include/hdd_krylov.hpp (a host-only C++ file that
+    links libhdd.so) into _build/libhdd_krylov_check.so."""
+    tmp = KRYLOV_CHECK_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", os.path.join(HERE, "..", "include"), KRYLOV_CHECK_SRC,
+           "-o", tmp, "-L", OUT, "-lhdd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("krylov adapter check failed to build:\n" + r.stderr[-4000:])
+    os.replace(tmp, KRYLOV_CHECK_LIB)
+    return KRYLOV_CHECK_LIB
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", "--offload-arch=gfx950", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lrocblas"]
+    cmd = [HIPCC, "-shared", "-fPIC", "-fopenmp", "--offload-arch=gfx950", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB)
+    build_krylov_check()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", KRYLOV_CHECK_LIB)
     return LIB
 
 

@@ -176,6 +176,28 @@ hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16) {
     HDD_TRY(ctx, ctx->solver->set_half_storage(fp16 != 0));
 }
 
+hdd_status hdd_set_sigma_refinement(hdd_ctx* ctx, int steps) {
+    if (!ctx) return HDD_ERR_ARG;
+    if (ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (steps < 0 || steps > 3) { ctx->err = "sigma refinement steps must be 0..3"; return HDD_ERR_ARG; }
+    HDD_TRY(ctx, ctx->solver->set_sigma_refinement(steps));
+}
+
+hdd_status hdd_set_pcg_precision(hdd_ctx* ctx, int fp64, int replace_every) {
+    if (!ctx) return HDD_ERR_ARG;
+    if (ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (replace_every < 0) { ctx->err = "replace_every must be >= 0"; return HDD_ERR_ARG; }
+    ctx->solver->outer_fp64 = fp64 != 0;
+    ctx->solver->replace_every = replace_every;
+    return HDD_OK;
+}
+
+hdd_status hdd_get_x_f64(hdd_ctx* ctx, int problem, double* host) {
+    if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
+    if (!ctx || !host || problem < 0 || problem >= ctx->solver->nprob()) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->download_x_f64(problem, host));
+}
+
 hdd_status hdd_destroy(hdd_ctx* ctx) {
     if (!ctx) return HDD_ERR_ARG;
     delete ctx;
@@ -257,6 +279,77 @@ hdd_status hdd_apply_preconditioner(hdd_ctx* ctx) {
     if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx) return HDD_ERR_ARG;
     HDD_TRY(ctx, { ctx->solver->apply_preconditioner(); hdd_check(hipStreamSynchronize(ctx->solver->stream), "apply"); })
+}
+
+// ---- Krylov plugin surface (KRYLOV_SYSTEM_BASE / CG_SYSTEM / CG_VECTOR)
+
+static bool dev_ctx(hdd_ctx* ctx) {
+    if (!ctx) return false;
+    if (ctx->solver->host_only) { ctx->err = "context was created host-only"; return false; }
+    return true;
+}
+static bool ch_ok(int c) { return c >= 0 && c <= 6; }
+
+hdd_status hdd_set_stream(hdd_ctx* ctx, void* stream) {
+    if (!dev_ctx(ctx)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->set_stream((hipStream_t)stream));
+}
+
+hdd_status hdd_synchronize(hdd_ctx* ctx) {
+    if (!dev_ctx(ctx)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, hdd_check(hipStreamSynchronize(ctx->solver->stream), "synchronize"));
+}
+
+hdd_status hdd_multiply(hdd_ctx* ctx, int ch_x, int ch_result) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_x) || !ch_ok(ch_result) || ch_x == ch_result) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->multiply(ch_x, ch_result));
+}
+
+hdd_status hdd_inner_product(hdd_ctx* ctx, int ch_a, int ch_b, double* out) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_a) || !ch_ok(ch_b) || !out) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->inner_product(ch_a, ch_b, out));
+}
+
+hdd_status hdd_convergence_norm(hdd_ctx* ctx, int ch, float* out) {
+    if (!dev_ctx(ctx) || !ch_ok(ch) || !out) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->convergence_norm(ch, out));
+}
+
+hdd_status hdd_project(hdd_ctx* ctx, int ch) {
+    if (!dev_ctx(ctx) || !ch_ok(ch)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->project(ch));
+}
+
+hdd_status hdd_vector_copy(hdd_ctx* ctx, int ch_out, float c, int ch_in) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_out) || !ch_ok(ch_in)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->vector_op(0, ch_out, ch_in, -1, c));
+}
+
+hdd_status hdd_vector_copy2(hdd_ctx* ctx, int ch_out, float c, int ch_in1, int ch_in2) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_out) || !ch_ok(ch_in1) || !ch_ok(ch_in2)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->vector_op(1, ch_out, ch_in1, ch_in2, c));
+}
+
+hdd_status hdd_vector_plus_equal(hdd_ctx* ctx, int ch_out, int ch_in) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_out) || !ch_ok(ch_in)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->vector_op(2, ch_out, ch_in, -1, 0.f));
+}
+
+hdd_status hdd_vector_minus_equal(hdd_ctx* ctx, int ch_out, int ch_in) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_out) || !ch_ok(ch_in)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->vector_op(3, ch_out, ch_in, -1, 0.f));
+}
+
+hdd_status hdd_vector_scale(hdd_ctx* ctx, int ch_out, float c) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_out)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->vector_op(4, ch_out, -1, -1, c));
+}
+
+hdd_status hdd_apply_preconditioner_channels(hdd_ctx* ctx, int ch_result, int ch_rhs, int ch_tmp) {
+    if (!dev_ctx(ctx) || !ch_ok(ch_result) || !ch_ok(ch_rhs) || !ch_ok(ch_tmp) || ch_result == ch_rhs || ch_result == ch_tmp ||
+        ch_rhs == ch_tmp)
+        return HDD_ERR_ARG;
+    HDD_TRY(ctx, ctx->solver->apply_preconditioner_channels(ch_result, ch_rhs, ch_tmp));
 }
 
 hdd_status hdd_sigma_info(const hdd_ctx* ctx, int* levels, int* nsub) {
@@ -460,6 +553,21 @@ static void plan_stats_impl(const SigmaHost& H, int level, int which, double* ou
 hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out, int cap, int* heights) {
     if (!ctx || !heights || cap < 0 || (cap > 0 && !out) || (which != 0 && which != 1)) return HDD_ERR_ARG;
     HDD_TRY(ctx, plan_stats_impl(ctx->solver->sigma_host(), level, which, out, cap, heights));
+}
+
+hdd_status hdd_sigma_factor_solve(hdd_ctx* ctx, int level, int which, int subdomain, const double* b, double* x) {
+    if (!ctx || !b || !x || (which != 0 && which != 1)) return HDD_ERR_ARG;
+    HDD_TRY(ctx, {
+        const SigmaHost& H = ctx->solver->sigma_host();
+        if (level < 0 || level >= H.levels) throw std::runtime_error("no such sigma level");
+        if (which == 1 && (subdomain < 0 || subdomain >= H.nsub)) throw std::runtime_error("no such subdomain");
+        ProblemHost& P = *ctx->solver->prob[0];
+        P.factorize();
+        const SigmaFactors::Level& F = P.factors->lv[level];
+        const SupernodalFactor& f = which == 0 ? F.Arr : F.Aii[F.rep[subdomain]];
+        if (which == 1 && H.Aii[level][subdomain].rows == 0) return HDD_OK;
+        factor_solve(f, b, x);
+    })
 }
 
 hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max) {

@@ -55,6 +55,10 @@ void launch_prolong(const DevLevel& F, const DevLevel& C, __half* u_f, const __h
 void launch_scale_to_half(const float* r, __half* hb, const int* page_prob, const float* scal, size_t n, hipStream_t s);
 void launch_unscale_project(const DevLevel& L, const __half* hz, float* z, const int* page_prob, const float* scal, hipStream_t s);
 void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s);
+// out = L v on Active|Interface cells only, other cells untouched (SPGrid_Laplace)
+void launch_laplace(const DevLevel& L, const float* v, float* out, hipStream_t s);
+// CG_VECTOR operations: 0 out = a*c, 1 out = a*c + b, 2 out += a, 3 out -= a, 4 out *= c
+void launch_vector_op(int op, float* out, const float* a, const float* b, float c, size_t n, hipStream_t s);
 void launch_project(const DevLevel& L, float* v, hipStream_t s);
 void launch_sub_project(const DevLevel& L, const float* b, const float* q, float* r, hipStream_t s);
 int reduce_parts(size_t n);
@@ -70,11 +74,17 @@ struct DevSegments {
     double* dpart = nullptr;          // [nprob*max_parts]
     float* fpart = nullptr;           // [nprob*max_parts]
     float* scal = nullptr;            // [nprob*8]
-    double* dscal = nullptr;          // [nprob*4]
+    double* dscal = nullptr;          // [nprob*8]
 };
 void launch_seg_copy2(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s);
 void launch_seg_xr_update(const DevSegments& S, float* x, float* r, const float* sv, const float* q, hipStream_t s);
 void launch_seg_dot(const DevSegments& S, const float* a, const float* b, int mode, hipStream_t s);
 void launch_seg_maxabs(const DevSegments& S, const float* a, hipStream_t s);
+// FP64 outer PCG: s-type update and x/r update with the double scalars, the true
+// residual b - L x from the double iterate, and the float copy of x
+void launch_seg_copy2_d(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s);
+void launch_seg_xr_update_d(const DevSegments& S, double* x, float* r, const float* sv, const float* q, hipStream_t s);
+void launch_true_residual(const DevLevel& L, const double* x, const float* b, float* r, hipStream_t s);
+void launch_cast_to_float(const double* x, float* y, size_t n, hipStream_t s);
 
 }  // namespace hdd

@@ -1,6 +1,9 @@
 #include "hdd_factor.h"
 #include "hdd_parallel.h"
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 #include <omp.h>
@@ -215,6 +218,8 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
     std::vector<std::vector<int>> by_h(F.max_height + 1);
     for (int s = 0; s < ns; ++s) by_h[F.sn[s].height].push_back(s);
     const int nthreads = omp_get_max_threads();
+    const bool use_device = device_fronts_enabled();
+    const int front_min = device_front_min_rows();
     auto front = [&](int s, bool inner_parallel, std::vector<int>& pos) {
         const Supernode& S = F.sn[s];
         const int nS = S.n, nR = (int)S.rows.size(), m = nS + nR;
@@ -240,7 +245,8 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
             }
             std::vector<double>().swap(U[c]);
         }
-        partial_cholesky(Fm.data(), m, nS, inner_parallel);
+        if (inner_parallel && m >= front_min && use_device) partial_cholesky_device(Fm.data(), m, nS);
+        else partial_cholesky(Fm.data(), m, nS, inner_parallel);
         std::vector<double> l11((size_t)nS * nS), l21((size_t)nR * nS);
         for (int j = 0; j < nS; ++j) {
             for (int i = j; i < nS; ++i) l11[(size_t)j * nS + i] = Fm[(size_t)j * m + i];
@@ -253,8 +259,10 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
         F.L21[s].swap(l21);
         U[s].swap(u);
     };
+    const bool verbose = std::getenv("HDD_FACTOR_VERBOSE") != nullptr;
     for (int h = 0; h <= F.max_height; ++h) {
         const auto& list = by_h[h];
+        auto th0 = std::chrono::steady_clock::now();
         if ((int)list.size() >= nthreads) {
             ParallelErrors errs;
 #pragma omp parallel
@@ -267,6 +275,12 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
         } else {
             std::vector<int> pos(n);
             for (int s : list) front(s, true, pos);
+        }
+        if (verbose && n > 20000) {
+            size_t mmax = 0; double flops = 0;
+            for (int s : list) { size_t m = F.sn[s].n + F.sn[s].rows.size(); mmax = std::max(mmax, m); flops += (double)F.sn[s].n * m * m; }
+            std::fprintf(stderr, "[factor n=%d] height %d: %zu fronts, max front %zu, ~%.2e flop, %.3fs\n", n, h, list.size(), mmax, flops,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count());
         }
     }
 }

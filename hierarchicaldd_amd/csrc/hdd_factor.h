@@ -36,6 +36,14 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
 // Host solve with the factor (double), for checks.
 void factor_solve(const SupernodalFactor& F, const double* b, double* x);
 
+// Dense fronts of at least device_front_min_rows() rows (kDeviceFrontMin, or
+// HDD_DEVICE_FRONT_MIN) are factored on the GPU (hdd_factor_device.cpp) when a
+// device is present and HDD_DEVICE_FRONTS is not 0.
+constexpr int kDeviceFrontMin = 1024;
+bool device_fronts_enabled();
+int device_front_min_rows();
+void partial_cholesky_device(double* F, int m, int ns);
+
 // In-place inverse of a dense lower-triangular n x n column-major matrix.
 void invert_lower(std::vector<double>& L, int n);
 

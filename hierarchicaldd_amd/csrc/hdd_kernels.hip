@@ -325,7 +325,9 @@ __global__ void __launch_bounds__(256) k_prolong(DevLevel F, DevLevel C, S* __re
     st(u_f, e, u);
 }
 
-// PCG operator: out = L v on Active|Interface cells (double accumulation), 0 elsewhere (fused Project)
+// PCG operator: out = L v on Active|Interface cells (double accumulation); kZero: 0 elsewhere
+// (the PCG's Multiply fused with Project), else other cells are left as they are (SPGrid_Laplace)
+template <bool kZero>
 __global__ void __launch_bounds__(256) k_laplace_pcg(DevLevel L, const float* __restrict__ v, float* __restrict__ out) {
     const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
     if (p >= L.np || page_stopped(L, p)) return;
@@ -338,8 +340,26 @@ __global__ void __launch_bounds__(256) k_laplace_pcg(DevLevel L, const float* __
         for (int k = 0; k < 6; ++k)
             if (f & (kFaceMX << k)) acc -= ((double)v[nb_index(L.nbr, p, c, k)] - cv);
         res = (float)acc;
+    } else if (!kZero) {
+        return;
     }
     out[e] = res;
+}
+
+// CG_VECTOR operations over the cells of the real pages (SPGrid_Gemini_Arithmetic_Helpers.h):
+// 0 Copy out = a*c, 1 Copy2 out = a*c + b, 2 Plus_Equal out += a, 3 Minus_Equal out -= a, 4 Scale out *= c
+__global__ void __launch_bounds__(256) k_vector_op(int op, float* __restrict__ out, const float* __restrict__ a, const float* __restrict__ b,
+                                                   float c, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        switch (op) {
+            case 0: out[i] = a[i] * c; break;
+            case 1: out[i] = a[i] * c + b[i]; break;
+            case 2: out[i] += a[i]; break;
+            case 3: out[i] -= a[i]; break;
+            default: out[i] *= c; break;
+        }
+    }
 }
 
 // Project: zero values outside Active|Interface
@@ -389,6 +409,55 @@ __global__ void __launch_bounds__(256) k_seg_xr_update(float* __restrict__ x, fl
     }
 }
 
+// FP64 outer variant of k_seg_copy2: out = (float)(in1 * dscal[slot] + in2) in double
+__global__ void __launch_bounds__(256) k_seg_copy2_d(float* __restrict__ out, const float* __restrict__ in1, const float* __restrict__ in2,
+                                                     const double* __restrict__ dscal, int slot, const int* __restrict__ page_prob,
+                                                     const int* __restrict__ active, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int pr = page_prob[i >> 8];
+        if (pr < 0 || !active[pr]) continue;
+        out[i] = (float)((double)in1[i] * dscal[pr * 8 + slot] + (double)in2[i]);
+    }
+}
+
+// FP64 outer variant of k_seg_xr_update: x (double) += alpha s; r = (float)(r - alpha q), alpha double
+__global__ void __launch_bounds__(256) k_seg_xr_update_d(double* __restrict__ x, float* __restrict__ r, const float* __restrict__ s,
+                                                         const float* __restrict__ q, const double* __restrict__ dscal,
+                                                         const int* __restrict__ page_prob, const int* __restrict__ active, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int pr = page_prob[i >> 8];
+        if (pr < 0 || !active[pr]) continue;
+        const double a = dscal[pr * 8 + 3];
+        x[i] += a * (double)s[i];
+        r[i] = (float)((double)r[i] - a * (double)q[i]);
+    }
+}
+
+// True residual r = b - L x from the FP64 iterate (double accumulation), 0 off Active|Interface
+__global__ void __launch_bounds__(256) k_true_residual(DevLevel L, const double* __restrict__ x, const float* __restrict__ b,
+                                                       float* __restrict__ r) {
+    const int p = xcd_page(L.np), c = threadIdx.x, e = p * 256 + c;
+    if (p >= L.np || page_stopped(L, p)) return;
+    const uint32_t f = L.flags[e];
+    float res = 0.f;
+    if (f & (kActive | kInterface)) {
+        const double cv = x[e];
+        double acc = (double)b[e];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (f & (kFaceMX << k)) acc += (x[nb_index(L.nbr, p, c, k)] - cv);
+        res = (float)acc;
+    }
+    r[e] = res;
+}
+
+__global__ void k_cast_to_float(const double* __restrict__ x, float* __restrict__ y, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = (float)x[i];
+}
+
 // block-partial double dot products of float products (a*b rounded to float, then summed in double);
 // block (c, p) = partial c of problem p
 __global__ void __launch_bounds__(256) k_seg_dot_partial(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ part,
@@ -423,20 +492,23 @@ __global__ void __launch_bounds__(256) k_seg_maxabs_partial(const float* __restr
 //   mode 0: rho = (float)dot -> scal[0]; beta = rho/rho_old -> scal[2]
 //   mode 1: s_dot_q = (float)dot; alpha = rho/s_dot_q -> scal[1]; rho_old = rho -> scal[3]
 //   mode 2: max-abs norm -> scal[4] (every problem; modes 0 and 1 active problems only)
+//   mode 3: the plain sum -> dscal[2] (every problem)
+//   modes 5 / 6: as 0 / 1 with the scalars in double (dscal: 0 rho, 1 s_dot_q,
+//   3 alpha, 4 beta, 5 rho_old), active problems only
 __global__ void __launch_bounds__(256) k_seg_finish(const double* __restrict__ dpart, const float* __restrict__ fpart,
                                                     const int* __restrict__ parts, int max_parts, int mode, const int* __restrict__ active,
                                                     float* __restrict__ scal_all, double* __restrict__ dscal_all) {
     const int p = blockIdx.x;
-    if (mode < 2 && !active[p]) return;
+    if ((mode == 0 || mode == 1 || mode == 5 || mode == 6) && !active[p]) return;
     const int nparts = parts[p];
     float* scal = scal_all + p * 8;
-    double* dscal = dscal_all + p * 4;
+    double* dscal = dscal_all + p * 8;
     __shared__ double sd[256];
     __shared__ float sf[256];
     double s = 0.0;
     float m = 0.f;
     for (int i = threadIdx.x; i < nparts; i += 256) {
-        if (mode < 2) s += dpart[(size_t)p * max_parts + i]; else m = fmaxf(m, fpart[(size_t)p * max_parts + i]);
+        if (mode != 2) s += dpart[(size_t)p * max_parts + i]; else m = fmaxf(m, fpart[(size_t)p * max_parts + i]);
     }
     sd[threadIdx.x] = s; sf[threadIdx.x] = m;
     __syncthreads();
@@ -450,6 +522,17 @@ __global__ void __launch_bounds__(256) k_seg_finish(const double* __restrict__ d
             scal[0] = rho;
             scal[2] = rho / scal[3];
             dscal[0] = sd[0];
+        } else if (mode == 3) {
+            dscal[2] = sd[0];  // a plain inner product (Inner_Product), no PCG scalar touched
+        } else if (mode == 5) {
+            // FP64 outer scalars: rho and beta = rho / rho_old in double
+            dscal[0] = sd[0];
+            dscal[4] = sd[0] / dscal[5];
+        } else if (mode == 6) {
+            // FP64 outer scalars: alpha = rho / s_dot_q, rho_old = rho
+            dscal[1] = sd[0];
+            dscal[3] = sd[0] != 0.0 ? dscal[0] / sd[0] : 3.40282347e+38;
+            dscal[5] = dscal[0];
         } else if (mode == 1) {
             const float sdq = (float)sd[0];
             const float rho = scal[0];
@@ -567,7 +650,13 @@ void launch_unscale_project(const DevLevel& L, const __half* hz, float* z, const
     if (L.np) hipLaunchKernelGGL(k_unscale_project, dim3(L.np + 1), dim3(256), 0, s, L, hz, z, page_prob, scal);
 }
 void launch_laplace_pcg(const DevLevel& L, const float* v, float* out, hipStream_t s) {
-    if (L.np) hipLaunchKernelGGL(k_laplace_pcg, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, v, out);
+    if (L.np) hipLaunchKernelGGL(k_laplace_pcg<true>, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, v, out);
+}
+void launch_laplace(const DevLevel& L, const float* v, float* out, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_laplace_pcg<false>, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, v, out);
+}
+void launch_vector_op(int op, float* out, const float* a, const float* b, float c, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_vector_op, dim3(grid_for(n)), dim3(256), 0, s, op, out, a, b, c, n);
 }
 void launch_project(const DevLevel& L, float* v, hipStream_t s) {
     if (L.np) hipLaunchKernelGGL(k_project, dim3(L.np), dim3(256), 0, s, L, v);
@@ -588,6 +677,20 @@ void launch_seg_dot(const DevSegments& S, const float* a, const float* b, int mo
     hipLaunchKernelGGL(k_seg_dot_partial, dim3(S.max_parts, S.nprob), dim3(256), 0, s, a, b, S.dpart, S.pg0, S.parts, S.max_parts);
     hipLaunchKernelGGL(k_seg_finish, dim3(S.nprob), dim3(256), 0, s, (const double*)S.dpart, (const float*)nullptr, S.parts, S.max_parts,
                        mode, S.active, S.scal, S.dscal);
+}
+void launch_seg_copy2_d(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_copy2_d, dim3(grid_for(S.ncells)), dim3(256), 0, s, out, in1, in2, (const double*)S.dscal, slot, S.page_prob,
+                       S.active, S.ncells);
+}
+void launch_seg_xr_update_d(const DevSegments& S, double* x, float* r, const float* sv, const float* q, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_xr_update_d, dim3(grid_for(S.ncells)), dim3(256), 0, s, x, r, sv, q, (const double*)S.dscal, S.page_prob,
+                       S.active, S.ncells);
+}
+void launch_true_residual(const DevLevel& L, const double* x, const float* b, float* r, hipStream_t s) {
+    if (L.np) hipLaunchKernelGGL(k_true_residual, dim3(kXcds * xcd_chunk(L.np)), dim3(256), 0, s, L, x, b, r);
+}
+void launch_cast_to_float(const double* x, float* y, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_cast_to_float, dim3(grid_for(n)), dim3(256), 0, s, x, y, n);
 }
 void launch_seg_maxabs(const DevSegments& S, const float* a, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_maxabs_partial, dim3(S.max_parts, S.nprob), dim3(256), 0, s, a, S.fpart, S.pg0, S.parts, S.max_parts);
@@ -643,3 +746,4 @@ void launch_if_scatter(const int* if_index, int n, const float* u0, __half* z, h
 }
 
 }  // namespace hdd
+

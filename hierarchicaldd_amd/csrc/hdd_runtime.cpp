@@ -267,7 +267,7 @@ void Solver::build_device() {
         seg_pg0.upload(pg0[0]); seg_parts.upload(parts); page_prob.upload(pp);
         active.upload(std::vector<int>(NP, 1));
         dpart.alloc((size_t)NP * max_parts); fpart.alloc((size_t)NP * max_parts);
-        scal.alloc((size_t)NP * 8); dscal.alloc((size_t)NP * 4);
+        scal.alloc((size_t)NP * 8); dscal.alloc((size_t)NP * 8);
     }
     // interface dof -> level-0 cell index (problems' dofs concatenated in problem order)
     {
@@ -291,8 +291,11 @@ void Solver::build_device() {
 
 Solver::~Solver() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (stream) (void)hipStreamSynchronize(stream);
     sigma.reset();
-    if (stream) (void)hipStreamDestroy(stream);
+    // a caller's stream (set_stream) is the caller's to destroy
+    hipStream_t mine = own_stream ? own_stream : stream;
+    if (mine) (void)hipStreamDestroy(mine);
 }
 
 void Solver::upload_levels() {
@@ -413,6 +416,104 @@ void Solver::set_half_storage(bool on) {
     half_storage = on;
 }
 
+void Solver::set_sigma_refinement(int steps) {
+    if (!sigma) throw std::runtime_error("no interface solver on this context");
+    std::vector<const SigmaHost*> hosts;
+    for (auto& q : prob) hosts.push_back(&q->sigma);
+    sigma->set_refinement(hosts, steps);
+    hdd_check(hipStreamSynchronize(stream), "sigma refinement setup");
+    // captured applies hold the previous launch sequence
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
+DBuf<float>& Solver::channel_buffer(int c) {
+    LevelBuffers& L0 = lv[0];
+    switch (c) {
+        case 0: return half_storage ? zf : L0.u[L0.cur];
+        case 1: return L0.b;
+        case 2: return L0.r;
+        case 3: return x;
+        case 4: return bvec;
+        case 5: return q;
+        case 6: return s;
+    }
+    throw std::runtime_error("bad channel id");
+}
+
+void Solver::multiply(int c_in, int c_out) {
+    launch_laplace(lv[0].view(), channel(c_in), channel(c_out), stream);
+}
+
+void Solver::inner_product(int a, int b, double* out) {
+    const DevSegments sg = segments();
+    launch_seg_dot(sg, channel(a), channel(b), 3, stream);
+    std::vector<double> d((size_t)nprob() * 8);
+    hdd_check(hipMemcpyAsync(d.data(), dscal.p, d.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "inner product");
+    hdd_check(hipStreamSynchronize(stream), "inner product");
+    for (int p = 0; p < nprob(); ++p) out[p] = d[(size_t)p * 8 + 2];
+}
+
+void Solver::convergence_norm(int c, float* out) {
+    launch_seg_maxabs(segments(), channel(c), stream);
+    std::vector<float> f((size_t)nprob() * 8);
+    hdd_check(hipMemcpyAsync(f.data(), scal.p, f.size() * sizeof(float), hipMemcpyDeviceToHost, stream), "convergence norm");
+    hdd_check(hipStreamSynchronize(stream), "convergence norm");
+    for (int p = 0; p < nprob(); ++p) out[p] = f[(size_t)p * 8 + 4];
+}
+
+void Solver::project(int c) { launch_project(lv[0].view(), channel(c), stream); }
+
+void Solver::vector_op(int op, int c_out, int c_a, int c_b, float scalar) {
+    const size_t n = (size_t)lv[0].host.npages() * kPageCells;
+    launch_vector_op(op, channel(c_out), op == 4 ? nullptr : channel(c_a), op == 1 ? channel(c_b) : nullptr, scalar, n, stream);
+}
+
+// The apply reads its right-hand side from channel 1 and leaves its result in
+// channel 0, using channel 2 and the other smoothing buffer as scratch. Other
+// channels are lent to it by exchanging buffers (all channels have the same
+// size and a zero last page), so no channel but result and tmp changes.
+void Solver::apply_preconditioner_channels(int c_result, int c_rhs, int c_tmp) {
+    for (int c : {c_result, c_rhs, c_tmp})
+        if (c < 0 || c > 6) throw std::runtime_error("bad channel id");
+    if (c_result == c_rhs || c_result == c_tmp || c_rhs == c_tmp) throw std::runtime_error("result, rhs and tmp must be three different channels");
+    if (c_result == 0 && c_rhs == 1 && c_tmp == 2) { apply_preconditioner(); return; }
+    // the exchanges below move buffers that captured applies have recorded
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+    LevelBuffers& L0 = lv[0];
+    auto swap_buf = [](DBuf<float>& a, DBuf<float>& b) { std::swap(a.p, b.p); };
+    // lend rhs as channel 1 and tmp as channel 2; a channel among {0, 1, 2} that is
+    // not in its role is parked in the buffer of the channel that takes its role
+    std::vector<std::pair<int, int>> done;
+    auto lend = [&](int role, int c) {
+        if (c == role) return;
+        swap_buf(channel_buffer(role), channel_buffer(c));
+        done.push_back({role, c});
+    };
+    lend(1, c_rhs);
+    lend(2, c_tmp == 1 ? c_rhs : c_tmp);
+    if (c_result != 0) swap_buf(channel_buffer(0), channel_buffer(c_result));
+    try {
+        record_apply();
+    } catch (...) {
+        if (c_result != 0) swap_buf(channel_buffer(0), channel_buffer(c_result));
+        for (auto it = done.rbegin(); it != done.rend(); ++it) swap_buf(channel_buffer(it->first), channel_buffer(it->second));
+        throw;
+    }
+    // the result is in channel 0's buffer now; give it to the result channel, channel 0 its own contents back
+    if (c_result != 0) swap_buf(channel_buffer(0), channel_buffer(c_result));
+    for (auto it = done.rbegin(); it != done.rend(); ++it) swap_buf(channel_buffer(it->first), channel_buffer(it->second));
+}
+
+void Solver::set_stream(hipStream_t s_new) {
+    hdd_check(hipStreamSynchronize(stream), "set_stream");
+    if (!own_stream) own_stream = stream;
+    stream = s_new ? s_new : own_stream;
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
 DevSegments Solver::segments() {
     DevSegments sg;
     sg.nprob = nprob(); sg.max_parts = max_parts; sg.ncells = (size_t)lv[0].host.npages() * kPageCells;
@@ -521,6 +622,15 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
     std::vector<float> hs((size_t)NP * 8, 0.f);
     for (int p = 0; p < NP; ++p) hs[(size_t)p * 8 + 3] = flt_max;  // rho_old
     std::vector<int> act(NP, 1);
+    const bool fp64 = outer_fp64;
+    if (fp64) {
+        if (xd.n != n0) xd.alloc(n0);
+        std::vector<double> hd((size_t)NP * 8, 0.0);
+        for (int p = 0; p < NP; ++p) hd[(size_t)p * 8 + 5] = flt_max;  // rho_old
+        hdd_check(hipMemcpyAsync(dscal.p, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        hdd_check(hipMemsetAsync(xd.p, 0, n0 * sizeof(double), stream));
+        hdd_check(hipStreamSynchronize(stream));  // hd goes out of scope
+    }
     hdd_check(hipMemcpyAsync(scal.p, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
     // while the batch iterates, the stencil kernels and exact solves skip the problems that have stopped
@@ -552,6 +662,19 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
         launch_seg_maxabs(sg, r, stream);
         hdd_check(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
         hdd_check(hipStreamSynchronize(stream));
+        if (fp64 && it > 0) {
+            // residual replacement: every replace_every iterations, and whenever a problem's
+            // recursive residual passes its test, r becomes the true residual b - L x of the
+            // double iterate, and the test is taken on that
+            bool replace = replace_every > 0 && it % replace_every == 0;
+            for (int p = 0; p < NP && !replace; ++p) replace = act[p] && hs[(size_t)p * 8 + 4] <= (float)tol[p];
+            if (replace) {
+                launch_true_residual(v0, xd.p, bvec.p, r, stream);
+                launch_seg_maxabs(sg, r, stream);
+                hdd_check(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+                hdd_check(hipStreamSynchronize(stream));
+            }
+        }
         const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         bool changed = false;
         for (int p = 0; p < NP; ++p) {
@@ -565,6 +688,12 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
             act[p] = 0;
             --n_active;
             changed = true;
+            if (fp64) {
+                // the float x channel of the stopped problem from its double iterate
+                const size_t first = (size_t)pg0[0][p] * kPageCells, cnt = (size_t)(pg0[0][p + 1] - pg0[0][p]) * kPageCells;
+                launch_cast_to_float(xd.p + first, x.p + first, cnt, stream);
+                hdd_check(hipStreamSynchronize(stream));
+            }
             if (on_stop) on_stop(p);
         }
         if (n_active == 0) break;
@@ -572,6 +701,15 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
         if (changed) hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
         apply_preconditioner();
         float* z = channel(0);
+        if (fp64) {
+            launch_seg_dot(sg, z, r, 5, stream);                   // rho, beta (double)
+            if (restart) launch_copy(s.p, z, n0, stream);
+            else launch_seg_copy2_d(sg, s.p, s.p, z, 4, stream);   // s = s*beta + z
+            launch_laplace_pcg(v0, s.p, q.p, stream);              // q = Project(A s)
+            launch_seg_dot(sg, s.p, q.p, 6, stream);               // alpha, rho_old (double)
+            launch_seg_xr_update_d(sg, xd.p, r, s.p, q.p, stream); // x (double) += alpha s; r -= alpha q
+            continue;
+        }
         launch_seg_dot(sg, z, r, 0, stream);                   // rho, beta
         if (restart) launch_copy(s.p, z, n0, stream);
         else launch_seg_copy2(sg, s.p, s.p, z, 2, stream);     // s = s*beta + z
@@ -590,6 +728,15 @@ int Solver::pcg(double tol_abs, int max_it, std::vector<double>& hist, double* s
     for (auto& r : res) worst = std::max(worst, r.seconds);
     if (seconds) *seconds = worst;
     return res[0].iterations;
+}
+
+void Solver::download_x_f64(int p, double* host) {
+    if (p < 0 || p >= nprob()) throw std::runtime_error("no such problem in the batch");
+    if (!xd.p) throw std::runtime_error("no FP64 iterate: the last solve ran with float outer scalars");
+    hdd_check(hipStreamSynchronize(stream), "sync");
+    const size_t first = (size_t)pg0[0][p] * kPageCells, n = (size_t)(pg0[0][p + 1] - pg0[0][p]) * kPageCells;
+    hdd_check(hipMemcpyAsync(host, xd.p + first, n * sizeof(double), hipMemcpyDeviceToHost, stream), "download x");
+    hdd_check(hipStreamSynchronize(stream), "download x");
 }
 
 void Solver::download_x(int p, float* host) {

@@ -155,8 +155,34 @@ public:
                    const std::function<void(int)>& on_stop = nullptr);
     // x of problem p (its level-0 pages in its own page order), n_pages(p)*256 floats
     void download_x(int p, float* host);
+    // FP64 outer PCG (opt-in; the reference keeps rho, alpha, beta in float): the
+    // scalars in double, x accumulated in double (xd; the x channel gets its
+    // float copy when the problem stops), and the true residual b - L x of the
+    // double iterate replacing r every replace_every iterations and whenever the
+    // recursive residual passes the test, which is then taken on the true one
+    bool outer_fp64 = false;
+    int replace_every = 25;
+    DBuf<double> xd;
+    void download_x_f64(int p, double* host);
     // V-cycle storage of the preconditioner: FP32 (the reference's) or FP16 with FP32 arithmetic
     void set_half_storage(bool on);
+    // steps of iterative refinement after every exact solve of the interface solver (0: none)
+    void set_sigma_refinement(int steps);
+
+    // Krylov plugin operations on level-0 channels (CG_SYSTEM / CG_VECTOR,
+    // DDS/CG_SYSTEM.cpp:29-144, DDS/CG_VECTOR.cpp:18-74), enqueued on the stream;
+    // the reductions wait for their results (one value per problem)
+    DBuf<float>& channel_buffer(int c);
+    void multiply(int c_in, int c_out);
+    void inner_product(int a, int b, double* out);
+    void convergence_norm(int c, float* out);
+    void project(int c);
+    void vector_op(int op, int c_out, int c_a, int c_b, float scalar);
+    // z = M^{-1} rhs then Project(z), into the result channel; tmp is scratch; no other channel changes
+    void apply_preconditioner_channels(int c_result, int c_rhs, int c_tmp);
+    // the stream every later call is enqueued on (nullptr: the context's own)
+    void set_stream(hipStream_t s);
+    hipStream_t own_stream = nullptr;
     DevSegments segments();
 
 private:

@@ -33,7 +33,15 @@ struct TriPlan {
     void build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, const std::vector<int>& prob_of = {});
     // per-problem flags (device, 1 = iterating): supernodes of stopped problems are skipped; nullptr: none skipped
     const int* active = nullptr;
+    // x = (A + eps I)^{-1} b; with refine > 0, each solve is followed by `refine`
+    // steps of iterative refinement, x += (A + eps I)^{-1} (b - (A + eps I) x), the
+    // residual accumulated in double from the plan's matrix refA (its own numbering)
     void solve(const float* b_orig, float* x_orig, hipStream_t s);
+    void solve_once(const float* b_orig, float* x_orig, hipStream_t s);
+    int refine = 0;
+    double refine_eps = 1e-5;
+    CsrDev refA;
+    DBuf<float> ref_r, ref_d;
     void fwd_height(int h, const float* b_orig, hipStream_t s);
     void bwd_height(int h, float* x_orig, hipStream_t s);
     // launch-shape autotuner (slice rows, contribution items per height); returns the choices changed
@@ -122,6 +130,9 @@ struct SigmaDevice {
     void vcycle(hipStream_t s, bool first);
     // per-problem flags of the batch PCG (device): exact solves skip stopped problems; nullptr: skip nothing
     void set_active(const int* active);
+    // steps of iterative refinement after every exact solve (0: none, the default)
+    int refine_steps = 0;
+    void set_refinement(const std::vector<const SigmaHost*>& hosts, int steps);
 };
 
 // Device interface solver of a batch: the problems' operators stacked

@@ -842,11 +842,38 @@ void TriPlan::top_solve(const float* b, float* x, hipStream_t s) {
                        (const float*)ty, (float*)nullptr, (const int*)orig_of.p, y.p + n, x, (const int*)sn_of.p, (const int*)sn_prob.p, active);
 }
 
-void TriPlan::solve(const float* b, float* x, hipStream_t s) {
+void TriPlan::solve_once(const float* b, float* x, hipStream_t s) {
     if (n == 0) return;
     for (int h = 0; h <= H; ++h) fwd_height(h, b, s);
     if (top_nq) top_solve(b, x, s);
     for (int h = H; h >= 0; --h) bwd_height(h, x, s);
+}
+
+// r = b - (A + eps I) x, accumulated in double (one thread per row, column order)
+__global__ void k_refine_residual(int rows, const int* __restrict__ p, const int* __restrict__ ix, const float* __restrict__ v,
+                                  double eps, const float* __restrict__ b, const float* __restrict__ x, float* __restrict__ r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    double acc = (double)b[i] - eps * (double)x[i];
+    for (int e = p[i]; e < p[i + 1]; ++e) acc -= (double)v[e] * (double)x[ix[e]];
+    r[i] = (float)acc;
+}
+
+__global__ void k_add_to(int n, const float* __restrict__ d, float* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+void TriPlan::solve(const float* b, float* x, hipStream_t s) {
+    solve_once(b, x, s);
+    if (n == 0 || refine <= 0) return;
+    if (refA.rows != n || ref_r.n < (size_t)n || ref_d.n < (size_t)n) throw std::runtime_error("refined solve: refinement operator not set up");
+    for (int k = 0; k < refine; ++k) {
+        hipLaunchKernelGGL(k_refine_residual, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const int*)refA.p.p, (const int*)refA.ix.p,
+                           (const float*)refA.v.p, refine_eps, b, (const float*)x, ref_r.p);
+        solve_once(ref_r.p, ref_d.p, s);
+        hipLaunchKernelGGL(k_add_to, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, (const float*)ref_d.p, x);
+    }
 }
 
 // Rebuilds the slice and item lists into the tuner's fixed-capacity device arrays.
@@ -1248,6 +1275,54 @@ void SigmaDevice::vcycle(hipStream_t s, bool first) {
                                (const float*)lv[l + 1].u.p, F.u.p);
         for (int i = 0; i < smoothing; ++i) smooth(l, s);
     }
+}
+
+// Iterative refinement of every exact solve (an accuracy mode; the reference's
+// single-precision PARDISO solves have no counterpart): the plans' matrices are
+// rebuilt from the host operators in the plans' numbering, problems and
+// subdomains in the order make_sigma_device lays them out.
+void SigmaDevice::set_refinement(const std::vector<const SigmaHost*>& hosts, int steps) {
+    if (steps < 0 || steps > 3) throw std::runtime_error("sigma refinement: steps must be 0..3");
+    const int NP = (int)hosts.size();
+    if (NP != nprob) throw std::runtime_error("sigma refinement: one host operator set per problem");
+    auto block_append = [](Csr& M, const Csr& A, int off) {
+        for (int r = 0; r < A.rows; ++r) {
+            for (int e = A.ptr[r]; e < A.ptr[r + 1]; ++e) { M.idx.push_back(A.idx[e] + off); M.val.push_back(A.val[e]); }
+            M.ptr.push_back((int)M.idx.size());
+        }
+        M.rows += A.rows;
+    };
+    if (steps > 0 && refine_steps == 0) {
+        for (int l = 0; l < levels; ++l) {
+            SigmaLevel& L = lv[l];
+            Csr ai, ar;
+            for (int p = 0; p < NP; ++p) {
+                const SigmaHost& H = *hosts[p];
+                for (int s = 0; s < H.nsub; ++s) block_append(ai, H.Aii[l][s], ai.rows);
+                block_append(ar, H.Arr[l], ar.rows);
+            }
+            ai.cols = ai.rows; ar.cols = ar.rows;
+            if (ai.rows != L.n_int || ar.rows != L.n_if) throw std::runtime_error("sigma refinement: operator sizes differ from the plans");
+            if (L.n_int) { L.Lii.refA.upload(ai); L.Lii.ref_r.alloc(L.n_int); L.Lii.ref_d.alloc(L.n_int); }
+            if (L.n_if) { L.Lrr.refA.upload(ar); L.Lrr.ref_r.alloc(L.n_if); L.Lrr.ref_d.alloc(L.n_if); }
+        }
+        if (bottom_direct && Lbottom.n > 0) {
+            const int lb = levels - 1;
+            Csr cb;
+            for (int p = 0; p < NP; ++p) {
+                if (hosts[p]->n_if[lb] == 0) continue;
+                Csr C; std::vector<Int3> xyz;
+                combined_bottom(*hosts[p], lb, C, xyz);
+                block_append(cb, C, cb.rows);
+            }
+            cb.cols = cb.rows;
+            if (cb.rows != Lbottom.n) throw std::runtime_error("sigma refinement: bottom system size differs from its plan");
+            Lbottom.refA.upload(cb); Lbottom.ref_r.alloc(cb.rows); Lbottom.ref_d.alloc(cb.rows);
+        }
+    }
+    refine_steps = steps;
+    for (auto& L : lv) { L.Lii.refine = steps; L.Lrr.refine = steps; }
+    Lbottom.refine = steps;
 }
 
 void SigmaDevice::set_active(const int* a) {

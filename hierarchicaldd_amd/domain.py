@@ -243,6 +243,55 @@ def config1():
     return d, DDConfig(sub_size=32, vlevels=4)
 
 
+def reference_domain_3d(n: int = 256, delta=(64, 64, 64), sub: int = 16, vlevels: int = 3):
+    """The reference 3D generator's own domain (GEN/Create_3D_Domain/main.cpp):
+    cells [0, n-1]^3; the Rasterizer (:72-152) marks cells outside the interior
+    [1, n-2]^3 Dirichlet and every interior cell Active except the Neumann
+    bubble, the cells whose centre (i - 0.5) * dX with dX = 1 / (n - 3) (a
+    GRID of n - 2 nodes on the unit box, :91-92) lies within 0.3 of (0.5, 0.5,
+    0.5): |c - 0.5|^2 <= 0.09 in float arithmetic (:121); Dirichlet cells with no
+    Active face neighbour are dropped (:132-139); a page is kept iff it holds an
+    Active or Dirichlet cell (:141-146); the right-hand side is a delta of
+    1/(dx*dx) * 4 = 4.0 at -position_delta (:287); face bits by Flaging (:296).
+    The generator's default -position_delta (64, 128, 128) lies inside the
+    bubble at n = 256 (|c - 0.5|^2 = 0.062), where its own assertion
+    flags(delta) != 0 (:284) fails, so the default here is (64, 64, 64), an
+    Active cell (the reference's 2D output was likewise made with an explicit
+    position)."""
+    f32 = np.float32
+    X, Y, Z = _padded(n)
+    dX = f32(1.0) / f32(n - 3)
+    ax = [((np.arange(m).astype(f32) - f32(0.5)) * dX - f32(0.5)) for m in (X, Y, Z)]
+    cx, cy, cz = ax[0][:, None, None], ax[1][None, :, None], ax[2][None, None, :]
+    bubble = ((cx * cx + cy * cy) + cz * cz) <= f32(0.09)
+    i = np.arange(X)[:, None, None]
+    j = np.arange(Y)[None, :, None]
+    k = np.arange(Z)[None, None, :]
+    interior = (i >= 1) & (i <= n - 2) & (j >= 1) & (j <= n - 2) & (k >= 1) & (k <= n - 2)
+    flags = np.zeros((X, Y, Z), dtype=np.uint32)
+    flags[~interior] = DIRICHLET
+    flags[interior & ~bubble] = ACTIVE
+    act = np.pad(flags == ACTIVE, 1)
+    has_act = (act[0:X, 1:Y + 1, 1:Z + 1] | act[2:X + 2, 1:Y + 1, 1:Z + 1] | act[1:X + 1, 0:Y, 1:Z + 1] |
+               act[1:X + 1, 2:Y + 2, 1:Z + 1] | act[1:X + 1, 1:Y + 1, 0:Z] | act[1:X + 1, 1:Y + 1, 2:Z + 2])
+    flags[(flags == DIRICHLET) & ~has_act] = 0
+    flags = _face_bits(flags)
+    dom = domain_from_flags(n, flags, delta=(tuple(int(v) for v in delta), 4.0),
+                            meta={"workload": f"refgen{n}_sub{sub}", "n": n, "generator": "GEN/Create_3D_Domain rules"})
+    return dom, DDConfig(sub_size=sub, vlevels=vlevels)
+
+
+def sphere_case(n: int, sub: int, vlevels: int, delta=None, sigma_levels: int = 3):
+    """The sphere workload at another size: n^3 cells, a sphere of radius n/4 at
+    the centre, a delta of 1.0 at ``delta`` (default (n/4, n/4, n/4)); sub_size
+    ``sub``, ``vlevels`` V-cycle levels. n = 64, sub 16, vlevels 3 is the 64^3
+    workload and n = 256, sub 32, vlevels 4 the 256^3 one."""
+    c = n // 2
+    d = delta if delta is not None else (n // 4, n // 4, n // 4)
+    dom = sphere_obstacle_box(n, [(c, c, c, n // 4)], delta=(tuple(d), 1.0), meta={"workload": f"sphere{n}_sub{sub}"})
+    return dom, DDConfig(sub_size=sub, vlevels=vlevels, sigma_levels=sigma_levels)
+
+
 def small_box(n: int = 32, sub: int = 16, vlevels: int = 3, seed: int = 0, sigma_levels: int = 3):
     """Small test problem: one sphere obstacle and a random RHS on Active cells."""
     rng = np.random.default_rng(seed)
@@ -288,27 +337,45 @@ def served_request(rng, nmin: int = 32, nmax: int = 384, sub: int = 32, vlevels:
             DDConfig(sub_size=sub, vlevels=vlevels))
 
 
-def sparse_spheres(n: int = 512, count: int = 24, rmin: float = 32, rmax: float = 96, seed: int = 6, sub: int = 32,
-                   vlevels: int = 4):
-    """Sparse domain in an n^3 box: Active cells are the union of ``count`` seeded
-    spheres (radius uniform in [rmin, rmax), centres uniform in the box) inside
-    the box interior; every other cell is zero Dirichlet (kept only next to an
-    Active cell, as the reference generator keeps Dirichlet cells). Right-hand
-    side i.i.d. N(0,1) on the Active cells with its mean removed. With 24
-    spheres of radius 32-96 in 512^3 the union covers 19-26% of the 4^3 blocks
-    over seeds 0-11; seed 6 gives the most (25.5%)."""
+def sparse_spheres(n: int = 512, count: int = 24, rmin: float = 32, rmax: float = 96, seed: int = 0, sub: int = 32,
+                   vlevels: int = 4, target_coverage: float | None = None):
+    """Sparse domain in an n^3 box: Active cells are the union of seeded spheres
+    (radius uniform in [rmin, rmax), centres uniform in the box) inside the box
+    interior; every other cell is zero Dirichlet (kept only next to an Active
+    cell, as the reference generator keeps Dirichlet cells). Right-hand side
+    i.i.d. N(0,1) on the Active cells with its mean removed.
+
+    ``count`` spheres are drawn; with ``target_coverage``, further spheres of
+    the same distribution are drawn until that fraction of the 4^3 blocks holds
+    an Active cell. (24 spheres of radius 32-96 in 512^3 cover 19-22% of the 4^3
+    blocks on average, less than the "about 30%" the 512^3 workload states, so
+    that workload draws until 30% and reports the sphere count.)"""
     rng = np.random.default_rng(seed)
     X, Y, Z = _padded(n)
     active = np.zeros((X, Y, Z), dtype=bool)
+    nb4 = n // 4
+    blocks4 = np.zeros((nb4, nb4, nb4), dtype=bool)
     spheres = []
-    for _ in range(count):
+
+    def add_sphere():
         r = float(rng.uniform(rmin, rmax))
         c = rng.uniform(0, n, size=3)
         spheres.append((float(c[0]), float(c[1]), float(c[2]), r))
         _mark_sphere(active, c[0], c[1], c[2], r)
-    active[:1] = active[n - 1:] = False
-    active[:, :1] = active[:, n - 1:] = False
-    active[:, :, :1] = active[:, :, n - 1:] = False
+        # Active cells stay inside the box interior [1, n-2]
+        active[:1] = active[n - 1:] = False
+        active[:, :1] = active[:, n - 1:] = False
+        active[:, :, :1] = active[:, :, n - 1:] = False
+        lo = [max(0, int(np.floor(v - r)) // 4) for v in c]
+        hi = [min(nb4, (int(np.ceil(v + r)) + 4) // 4) for v in c]
+        if all(h > l for l, h in zip(lo, hi)):
+            sl = active[lo[0] * 4:hi[0] * 4, lo[1] * 4:hi[1] * 4, lo[2] * 4:hi[2] * 4]
+            blocks4[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = sl.reshape(hi[0] - lo[0], 4, hi[1] - lo[1], 4, hi[2] - lo[2], 4).any(axis=(1, 3, 5))
+
+    for _ in range(count):
+        add_sphere()
+    while target_coverage is not None and blocks4.mean() < target_coverage and len(spheres) < 10 * count + 100:
+        add_sphere()
     flags = np.zeros((X, Y, Z), dtype=np.uint32)
     flags[active] = ACTIVE
     pad = np.pad(active, 1)
@@ -320,9 +387,9 @@ def sparse_spheres(n: int = 512, count: int = 24, rmin: float = 32, rmax: float 
     act = active[:n, :n, :n]
     rhs[act] -= rhs[act].mean()
     rhs[~act] = 0
-    blocks4 = active[:n - n % 4, :n - n % 4, :n - n % 4].reshape(n // 4, 4, n // 4, 4, n // 4, 4).any(axis=(1, 3, 5)).mean()
+    cov4 = active[:n - n % 4, :n - n % 4, :n - n % 4].reshape(n // 4, 4, n // 4, 4, n // 4, 4).any(axis=(1, 3, 5)).mean()
     d = domain_from_flags(n, flags, rhs=rhs.astype(np.float32),
-                          meta={"workload": f"sparse{n}_spheres{count}", "n": n, "spheres": spheres,
-                                "active_cells": int(active.sum()), "coverage_4cubed_blocks": float(blocks4)})
+                          meta={"workload": f"sparse{n}_spheres{len(spheres)}", "n": n, "spheres": spheres, "sphere_count": len(spheres),
+                                "active_cells": int(active.sum()), "coverage_4cubed_blocks": float(cov4)})
     d.meta["coverage_pages"] = d.n_pages / float(np.prod([-(-n // b) for b in (BX, BY, BZ)]))
     return d, DDConfig(sub_size=sub, vlevels=vlevels)

@@ -54,7 +54,11 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats",
            "hdd_profile_solve", "hdd_create_batch", "hdd_batch_count", "hdd_batch_num_pages", "hdd_batch_page_offsets",
            "hdd_batch_pcg_solve", "hdd_batch_get_x", "hdd_create_batch_from", "hdd_server_create", "hdd_server_submit",
-           "hdd_server_wait", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph", "hdd_set_vcycle_storage"]
+           "hdd_server_wait", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph", "hdd_set_vcycle_storage",
+           "hdd_set_sigma_refinement", "hdd_sigma_factor_solve", "hdd_set_stream", "hdd_synchronize", "hdd_multiply",
+           "hdd_inner_product", "hdd_convergence_norm", "hdd_project", "hdd_vector_copy", "hdd_vector_copy2",
+           "hdd_vector_plus_equal", "hdd_vector_minus_equal", "hdd_vector_scale", "hdd_apply_preconditioner_channels",
+           "hdd_set_pcg_precision", "hdd_get_x_f64"]
 
 
 def lib():
@@ -109,6 +113,23 @@ def lib():
         L.hdd_server_budget.argtypes = [P, C.POINTER(C.c_double)]
         L.hdd_set_graph.argtypes = [P, I]
         L.hdd_set_vcycle_storage.argtypes = [P, I]
+        L.hdd_set_sigma_refinement.argtypes = [P, I]
+        L.hdd_sigma_factor_solve.argtypes = [P, I, I, I, P, P]
+        F = C.c_float
+        L.hdd_set_stream.argtypes = [P, P]
+        L.hdd_synchronize.argtypes = [P]
+        L.hdd_multiply.argtypes = [P, I, I]
+        L.hdd_inner_product.argtypes = [P, I, I, P]
+        L.hdd_convergence_norm.argtypes = [P, I, P]
+        L.hdd_project.argtypes = [P, I]
+        L.hdd_vector_copy.argtypes = [P, I, F, I]
+        L.hdd_vector_copy2.argtypes = [P, I, F, I, I]
+        L.hdd_vector_plus_equal.argtypes = [P, I, I]
+        L.hdd_vector_minus_equal.argtypes = [P, I, I]
+        L.hdd_vector_scale.argtypes = [P, I, F]
+        L.hdd_apply_preconditioner_channels.argtypes = [P, I, I, I]
+        L.hdd_set_pcg_precision.argtypes = [P, I, I]
+        L.hdd_get_x_f64.argtypes = [P, I, P]
         for name in EXPORTS:
             if name not in ("hdd_last_error", "hdd_server_last_error"):
                 getattr(L, name).restype = I
@@ -231,6 +252,13 @@ class DDSolver:
                                                _ptr(ptr), _ptr(idx), _ptr(val)))
         return r.value, c.value, ptr, idx, val
 
+    def sigma_factor_solve(self, level, which, sub, b):
+        """(A + 1e-5 I)^{-1} b in double with the host factor of Arr or of subdomain sub's Aii."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        self._check(lib().hdd_sigma_factor_solve(self.h, level, {"Arr": 0, "Aii": 1}[which], sub, _ptr(b), _ptr(x)))
+        return x
+
     def sigma_vcycle(self, b):
         b = np.ascontiguousarray(b, dtype=np.float32)
         u = np.zeros_like(b)
@@ -250,6 +278,20 @@ class DDSolver:
     def set_fp16(self, enable: bool):
         """FP16 storage (FP32 arithmetic) for the V-cycle vectors of the preconditioner."""
         self._check(lib().hdd_set_vcycle_storage(self.h, int(bool(enable))))
+
+    def set_outer_fp64(self, enable: bool, replace_every: int = 25):
+        """FP64 PCG scalars and iterate with true-residual replacement (default: the reference's float scalars)."""
+        self._check(lib().hdd_set_pcg_precision(self.h, int(bool(enable)), int(replace_every)))
+
+    def x_f64(self, problem: int = 0):
+        """The FP64 iterate of the last FP64-outer solve, [n_pages, 256] in page order."""
+        out = np.zeros((self.n_pages, 256), dtype=np.float64)
+        self._check(lib().hdd_get_x_f64(self.h, problem, _ptr(out)))
+        return out
+
+    def set_sigma_refinement(self, steps: int):
+        """Iterative-refinement steps after every exact solve of the interface solver (0: none, default)."""
+        self._check(lib().hdd_set_sigma_refinement(self.h, int(steps)))
 
     def solve(self, tolerance: float, max_iterations: int = 5000):
         """PCG from x=0; returns (iterations, residual history, seconds)."""

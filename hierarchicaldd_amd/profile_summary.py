@@ -12,7 +12,12 @@ of a kernel whose name contains KERNEL on are counted (e.g. k_laplace_pcg: the
 solves, without the set-up).
 With --pmc the input is the *counter_collection.csv of a `rocprofv3 --pmc`
 run (written with --output-format csv): the mean value of each counter per
-kernel and grid size is printed.
+kernel and grid size is printed. With --pmc --per-solve N --copy-bytes B
+(the run of `bench.py --plan-only`: calibration copies of B bytes read and B
+written, then N solves of one triangular-solve plan) the counter is also
+summed over every dispatch after the last k_copy and divided by N, and the
+copy's mean value is printed beside B, so that the plan's HBM bytes per solve
+are the sum scaled by B over the copy's value.
 """
 from __future__ import annotations
 
@@ -83,10 +88,39 @@ def pmc_rows(path):
     return acc
 
 
+def per_solve_lines(files, n_solves, copy_bytes):
+    rows = []
+    for fpath in files:
+        with open(fpath) as f:
+            for r in csv.DictReader(f):
+                rows.append((int(r["Dispatch_Id"]), short(r["Kernel_Name"]), r["Counter_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    copies = [i for i, (d, n, c, v) in enumerate(rows) if n.endswith("k_copy")]
+    if not copies:
+        raise SystemExit("no k_copy dispatch: --per-solve needs the calibration copies before the solves")
+    last = copies[-1]
+    copy_vals = [v for d, n, c, v in rows if n.endswith("k_copy")]
+    after = rows[last + 1:]
+    total = sum(v for d, n, c, v in after)
+    kernels = {}
+    for d, n, c, v in after:
+        k = kernels.setdefault(n, [0, 0.0])
+        k[0] += 1
+        k[1] += v
+    lines = [f"per-solve KiB: {total / n_solves:.1f}", f"copy KiB: {sum(copy_vals) / len(copy_vals):.1f}", f"copy bytes: {copy_bytes:.0f}",
+             f"solves: {n_solves}, dispatches after the last copy: {len(after)}",
+             f"{'kernel':48s} {'dispatches/solve':>16s} {'KiB/solve':>12s}"]
+    for n, (c, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"{n[:48]:48s} {c / n_solves:16.1f} {t / n_solves:12.1f}")
+    return lines
+
+
 def main_pmc(a):
     files = glob.glob(os.path.join(a.dir, "**", "*counter_collection.csv"), recursive=True)
     if not files:
         raise SystemExit(f"no counter_collection.csv under {a.dir}")
+    if a.per_solve:
+        return ([f"command: {a.cmd}"] if a.cmd else []) + per_solve_lines(files, a.per_solve, a.copy_bytes)
     acc = {}
     for fpath in files:
         for k, (c, t) in pmc_rows(fpath).items():
@@ -108,6 +142,8 @@ def main():
     ap.add_argument("--top", type=int, default=0, help="print only the N largest rows")
     ap.add_argument("--pmc", action="store_true")
     ap.add_argument("--from", dest="first", default=None, help="count dispatches from the first launch of this kernel on")
+    ap.add_argument("--per-solve", type=int, default=0, help="with --pmc: counter sum per solve after the last k_copy")
+    ap.add_argument("--copy-bytes", type=float, default=0.0, help="with --per-solve: bytes read (and written) by one calibration copy")
     a = ap.parse_args()
     if a.pmc:
         text = "\n".join(main_pmc(a)) + "\n"

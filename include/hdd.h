@@ -128,6 +128,32 @@ hdd_status hdd_set_graph(hdd_ctx* ctx, int enable);
  * FP16 buffers; hdd_vcycle stays FP32. */
 hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16);
 
+/* Accuracy mode of the interface solver (no reference counterpart): steps = 0
+ * (default) solves every (A + 1e-5 I) system of SIGMA_SMOOTHER::Smooth /
+ * Compute_Residual (IFS/SIGMA_SMOOTHER.cpp:292-352) once with FP32 factors, the
+ * precision of the reference's single-precision PARDISO (iparm[27] = 1,
+ * IFS/SIGMA_SMOOTHER.cpp:151); steps = 1..3 follows each solve with that many
+ * steps of iterative refinement, x += (A + 1e-5 I)^{-1} (b - (A + 1e-5 I) x),
+ * the residual accumulated in FP64, so the solves are accurate to FP32
+ * rounding of their result (each step costs one more solve). */
+hdd_status hdd_set_sigma_refinement(hdd_ctx* ctx, int steps);
+
+/* Precision of the PCG loop (no reference counterpart): fp64 = 0 (default)
+ * keeps CONJUGATE_GRADIENT_DUMPING's float rho, s.q, alpha and beta
+ * (DDS/CONJUGATE_GRADIENT_DUMPING.cpp:193-205), with its true residual only at
+ * the start (:156-162); fp64 = 1 forms those scalars in FP64, accumulates x in
+ * FP64 (the x channel receives its float copy), and replaces the recursive
+ * residual by the true residual b - L x of the FP64 iterate every
+ * `replace_every` iterations (0: never on a schedule) and whenever it passes
+ * the stopping test, which is then taken on the true residual, so a solve
+ * that stops has max|b - L x| <= tolerance. Vectors r, s, q, z and the
+ * preconditioner stay FP32. */
+hdd_status hdd_set_pcg_precision(hdd_ctx* ctx, int fp64, int replace_every);
+
+/* The FP64 iterate of the last solve with fp64 = 1: n_pages(problem)*256
+ * doubles in the problem's page order. */
+hdd_status hdd_get_x_f64(hdd_ctx* ctx, int problem, double* host);
+
 /* Releases all device and host memory of a context. */
 hdd_status hdd_destroy(hdd_ctx* ctx);
 
@@ -166,6 +192,67 @@ hdd_status hdd_vcycle(hdd_ctx* ctx);
  * Preconditioner::Apply_Preconditioner (DDS/SPGrid_Domain_Decomposition_Preconditioner.h:88-128)
  * as called through KRYLOV_SYSTEM_BASE::Precondition. */
 hdd_status hdd_apply_preconditioner(hdd_ctx* ctx);
+
+/* ---- Krylov plugin surface on device channels. The reference's PCG
+ * (CONJUGATE_GRADIENT_DUMPING::Solve, DDS/CONJUGATE_GRADIENT_DUMPING.cpp:146-212)
+ * drives its problem through KRYLOV_SYSTEM_BASE (PhysBAM_Tools/Krylov_Solvers/
+ * KRYLOV_SYSTEM_BASE.h:15-43), implemented by CG_SYSTEM (DDS/CG_SYSTEM.cpp:29-144),
+ * and KRYLOV_VECTOR_BASE, implemented by CG_VECTOR (DDS/CG_VECTOR.cpp:18-74), whose
+ * vectors are SPGrid channels. The entry points below are those operations on the
+ * context's level-0 channels (HDD_CH_*), resident in device memory: a Krylov
+ * loop written against them moves no vector across PCIe. They are enqueued on
+ * the context's stream (hdd_set_stream) and return before the work is done,
+ * except hdd_inner_product and hdd_convergence_norm, which wait for their
+ * values. On a batch context each reduction returns one value per problem and
+ * a scalar argument applies to every problem. include/hdd_krylov.hpp wraps them
+ * in CG_SYSTEM- and CG_VECTOR-shaped C++ classes. */
+
+/* Work of later calls goes to `stream` (a hipStream_t; NULL: the context's
+ * own non-blocking stream, the default). Waits for the work already queued on
+ * the previous stream. The caller keeps ownership of its stream. */
+hdd_status hdd_set_stream(hdd_ctx* ctx, void* stream);
+
+/* Waits for all work queued on the context's stream. */
+hdd_status hdd_synchronize(hdd_ctx* ctx);
+
+/* result = L x on Active|Interface cells, double accumulation, other cells
+ * unchanged. Replaces CG_SYSTEM::Multiply -> SPGrid_Laplace
+ * (DDS/CG_SYSTEM.cpp:29-44, DDS/SPGrid_Gemini_Arithmetic_Helpers.h:210-263). */
+hdd_status hdd_multiply(hdd_ctx* ctx, int ch_x, int ch_result);
+
+/* out[p] = sum over every cell of problem p of (float)(a*b), summed in double.
+ * Replaces CG_SYSTEM::Inner_Product (DDS/CG_SYSTEM.cpp:50-73). */
+hdd_status hdd_inner_product(hdd_ctx* ctx, int ch_a, int ch_b, double* out);
+
+/* out[p] = max |v| over the cells of problem p. Replaces
+ * CG_SYSTEM::Convergence_Norm (DDS/CG_SYSTEM.cpp:78-96). */
+hdd_status hdd_convergence_norm(hdd_ctx* ctx, int ch, float* out);
+
+/* Zero every cell that is neither Active nor Interface. Replaces
+ * CG_SYSTEM::Project (DDS/CG_SYSTEM.cpp:101-119). (Set_Boundary_Conditions and
+ * Project_Nullspace are no-ops in CG_SYSTEM, :124-133, and have no entry point.) */
+hdd_status hdd_project(hdd_ctx* ctx, int ch);
+
+/* CG_VECTOR operations (DDS/CG_VECTOR.cpp:18-74 -> SPGrid_Copy, SPGrid_Copy2,
+ * SPGrid_Plus_Equal, SPGrid_Minus_Equal, SPGrid_Scale of
+ * DDS/SPGrid_Gemini_Arithmetic_Helpers.h:31-207), over every cell:
+ *   copy: out = in * c        copy2: out = in1 * c + in2
+ *   plus_equal: out += in     minus_equal: out -= in     scale: out *= c */
+hdd_status hdd_vector_copy(hdd_ctx* ctx, int ch_out, float c, int ch_in);
+hdd_status hdd_vector_copy2(hdd_ctx* ctx, int ch_out, float c, int ch_in1, int ch_in2);
+hdd_status hdd_vector_plus_equal(hdd_ctx* ctx, int ch_out, int ch_in);
+hdd_status hdd_vector_minus_equal(hdd_ctx* ctx, int ch_out, int ch_in);
+hdd_status hdd_vector_scale(hdd_ctx* ctx, int ch_out, float c);
+
+/* result = M^{-1} rhs followed by Project(result): the DD preconditioner
+ * with its result, rhs and tmp fields chosen by channel, as
+ * SPGrid_Domain_Decomposition_Preconditioner::Apply_Preconditioner(result, rhs, tmp)
+ * (DDS/SPGrid_Domain_Decomposition_Preconditioner.h:88-128) called through
+ * CG_SYSTEM::Apply_Preconditioner (DDS/CG_SYSTEM.cpp:138-144) and
+ * KRYLOV_SYSTEM_BASE::Precondition. The three channels must differ; tmp is
+ * overwritten; every other channel keeps its contents. hdd_apply_preconditioner
+ * is this call with (HDD_CH_Z, HDD_CH_R, HDD_CH_K), synchronous. */
+hdd_status hdd_apply_preconditioner_channels(hdd_ctx* ctx, int ch_result, int ch_rhs, int ch_tmp);
 
 /* Interface solver introspection: sigma levels, subdomains, interface dofs. */
 hdd_status hdd_sigma_info(const hdd_ctx* ctx, int* levels, int* n_subdomains);
@@ -249,6 +336,13 @@ hdd_status hdd_profile_solve(hdd_ctx* ctx, int level, int which, int reps, doubl
  * the first cap/5 are written). Used to state the algorithmic bytes of the
  * solve kernels. No reference counterpart. */
 hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out, int cap, int* heights);
+
+/* x = (A + 1e-5 I)^{-1} b in double precision with the host factor of sigma
+ * level `level` (which = 0: Arr, 1: Aii of `subdomain`), the factorisation
+ * that replaces PARDISO phases 11/22 (IFS/SIGMA_SMOOTHER.cpp:115-288); b and x
+ * hold the matrix's rows (hdd_interface_matrix numbering). Works on host-only
+ * contexts. For checking the factors; no reference counterpart. */
+hdd_status hdd_sigma_factor_solve(hdd_ctx* ctx, int level, int which, int subdomain, const double* b, double* x);
 
 /* Eigenvalues >= lambda_cut (0 < lambda_cut < 1) of the bottom sigma
  * smoother's iteration matrix (Arr+eI)^-1 sum_s Air_s^T (Aii_s+eI)^-1 Air_s,

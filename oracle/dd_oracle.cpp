@@ -14,6 +14,7 @@
 #include "dd_sigma.h"
 #include "dd_vcycle.h"
 #include <chrono>
+#include <omp.h>
 #include <cstdlib>
 #include <cstdio>
 #include <memory>
@@ -26,6 +27,7 @@ enum { CZ = 0, CRr = 1, CK = 2, CX = 3, CB = 4, CQ = 5, CS = 6 };
 struct Params {
     int size[3], sub[3];
     int vlevels, sigma_levels, nV, int_s, bnd_s, sig_s, sig_mg;
+    bool sigma_single;  // single-precision sigma solves (factor rounded to float, float substitution)
 };
 
 template <class T>
@@ -66,7 +68,7 @@ struct Solver {
         lap("interface operators");
         domain_divider(g, size, sub);
         flaging(g);
-        mg.initialize(S, P.sig_s);
+        mg.initialize(S, P.sig_s, P.sigma_single);
         lap("factorisation");
         build_topology(topo, g, size, P.vlevels);
         lap("topology");
@@ -225,15 +227,17 @@ extern "C" {
 // Builds the solver and runs the full setup. Returns nullptr on failure
 // (message in *err_out if given). Arrays follow the reference file format:
 // page_offsets are SPGrid linear offsets of 4 KiB pages; flags / b hold 256
-// values per page in page-local (x,y,z) order.
+// values per page in page-local (x,y,z) order. sigma_single selects the
+// single-precision sigma solves (dd_sigma.h), 0 the exact (double) ones.
 void* ora_create(int is_double, int nb, const uint64_t* page_offsets, const uint32_t* flags, const float* b,
                  const int* size3, const int* sub3, int vlevels, int sigma_levels, int nV, int int_s, int bnd_s,
-                 int sig_s, int sig_mg, char* err_out, int err_len) {
+                 int sig_s, int sig_mg, int sigma_single, char* err_out, int err_len) {
     auto* h = new OraHandle();
     h->is_double = is_double;
     try {
         auto init = [&](auto& s) {
-            s.P = Params{{size3[0], size3[1], size3[2]}, {sub3[0], sub3[1], sub3[2]}, vlevels, sigma_levels, nV, int_s, bnd_s, sig_s, sig_mg};
+            s.P = Params{{size3[0], size3[1], size3[2]}, {sub3[0], sub3[1], sub3[2]}, vlevels, sigma_levels, nV, int_s, bnd_s, sig_s, sig_mg,
+                         sigma_single != 0};
             s.size = {size3[0], size3[1], size3[2]};
             s.sub = {sub3[0], sub3[1], sub3[2]};
             s.load(nb, page_offsets, flags, b);
@@ -250,6 +254,10 @@ void* ora_create(int is_double, int nb, const uint64_t* page_offsets, const uint
 }
 
 void ora_destroy(void* hp) { delete (OraHandle*)hp; }
+
+// OpenMP threads of later oracle calls (the reference's -threads N)
+void ora_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+int ora_max_threads() { return omp_get_max_threads(); }
 
 double ora_setup_seconds(void* hp) { auto* h = (OraHandle*)hp; return ORA_DISPATCH(h, s.t_setup); }
 
@@ -347,6 +355,53 @@ void ora_sigma_smooth(void* hp, int l, const double* b, double* u) {
         for (size_t i = 0; i < n; ++i) { bb[i] = (TT)b[i]; uu[i] = (TT)u[i]; }
         mg.smooth(l, uu, tmp, bb);
         for (size_t i = 0; i < n; ++i) u[i] = (double)uu[i];
+        return 0; })());
+}
+
+// STRIDE_GENERATOR::Generate_Adaptive_Stride for one axis over [lo, hi]:
+// q[0 .. hi-lo] receives the strides, *level the adaptation level.
+void ora_adaptive_stride(int lo, int hi, int* q, int* level) {
+    std::vector<int> v;
+    *level = adaptive_stride_axis(lo, hi, v);
+    for (size_t i = 0; i < v.size(); ++i) q[i] = v[i];
+}
+
+// channel c (0 u, 1 b, 2 r) of V-cycle level l in block order (level 0: the
+// main grid's z, r, k)
+void ora_level_get(void* hp, int l, int c, double* out) {
+    auto* h = (OraHandle*)hp;
+    ORA_DISPATCH(h, ([&] { auto& G = *s.topo.lv[l]; size_t n = 0; for (int p : G.blocks) for (int k = 0; k < BCELLS; ++k) out[n++] = (double)G.ch[c][(size_t)p * BCELLS + k]; return 0; })());
+}
+void ora_level_set(void* hp, int l, int c, const double* in) {
+    auto* h = (OraHandle*)hp;
+    ORA_DISPATCH(h, ([&] {
+        auto& G = *s.topo.lv[l];
+        using TT = typename std::remove_reference<decltype(G.ch[0][0])>::type;
+        size_t n = 0;
+        for (int p : G.blocks) for (int k = 0; k < BCELLS; ++k) G.ch[c][(size_t)p * BCELLS + k] = (TT)in[n++];
+        return 0; })());
+}
+// single V-cycle kernels on level l: 0 residual, 1 correction (mode 0 interior,
+// 1 boundary, 2 all Active), 2 restriction l -> l+1, 3 prolongation l+1 -> l
+void ora_level_kernel(void* hp, int l, int kernel, int mode) {
+    auto* h = (OraHandle*)hp;
+    ORA_DISPATCH(h, ([&] {
+        auto& T = s.topo;
+        if (kernel == 0) vc_residual(*T.lv[l]);
+        else if (kernel == 1) vc_correction(*T.lv[l], mode == 0 || mode == 2, mode == 1 || mode == 2);
+        else if (kernel == 2) vc_restriction(*T.lv[l], *T.lv[l + 1]);
+        else vc_prolongation(*T.lv[l], *T.lv[l + 1]);
+        return 0; })());
+}
+// x = (A + 1e-5 I)^{-1} b with the sigma solver's factor of level l
+// (which 0: Arr, 1: Aii of subdomain sd), in the oracle's solve precision
+void ora_sigma_factor_solve(void* hp, int l, int which, int sd, const double* b, double* x) {
+    auto* h = (OraHandle*)hp;
+    ORA_DISPATCH(h, ([&] {
+        const Cholesky& F = which == 0 ? s.mg.Lrr[l] : s.mg.Lii[l][sd];
+        std::vector<double> bb(b, b + F.n), xx(F.n);
+        F.solve(bb.data(), xx.data());
+        for (int i = 0; i < F.n; ++i) x[i] = xx[i];
         return 0; })());
 }
 

@@ -5,10 +5,17 @@
 //   SIGMA_SMOOTHER::Compute_Residual  IFS/SIGMA_SMOOTHER.cpp:328-352
 //   Eigen_To_Pardiso (+1e-5 diagonal) IFS/SIGMA_SMOOTHER.cpp:50-111
 // The reference solves (A + 1e-5 I) x = b with MKL PARDISO (METIS ordering,
-// single precision). MKL is absent here; this oracle factors A + 1e-5 I with
-// its own double-precision sparse Cholesky (nested-dissection ordering from
-// BFS level structures) and rounds the solution to T. PARDISO's own rounding
-// is therefore not reproduced ("parity unpinned" at the rounding level).
+// single precision: iparm[27] = 1 for float, IFS/SIGMA_SMOOTHER.cpp:151). MKL is
+// absent here; this oracle factors A + 1e-5 I with its own sparse Cholesky
+// (nested-dissection ordering from BFS level structures) in double and has two
+// precisions for the solves:
+//   * double (default): the double factor, double substitution, the solution
+//     rounded to T: the exact solve of the shifted system up to T rounding;
+//   * single: the factor rounded to float and float substitution, the storage
+//     and arithmetic precision of single-precision PARDISO. PARDISO's own
+//     ordering and rounding are not reproduced ("parity unpinned" at the
+//     rounding level); this mode measures how far single-precision solves move
+//     results from the exact ones.
 // Eigen SpMV orders follow Eigen's kernels: row-major products accumulate each
 // row in column order; transposed (scatter) products accumulate into the
 // destination in source-row order.
@@ -107,6 +114,7 @@ struct Cholesky {
     std::vector<int> perm, iperm;      // perm: new -> old
     std::vector<int> Lp, Li;           // CSC, diagonal first in each column
     std::vector<double> Lx;
+    bool single = false;               // factor rounded to float, float substitution
 
     template <class T>
     void factor(const CSR<T>& A, double eps) {
@@ -170,9 +178,11 @@ struct Cholesky {
             if (!(d > 0)) throw std::runtime_error("Cholesky: matrix not positive definite");
             Li[Lp[k]] = k; Lx[Lp[k]] = std::sqrt(d);
         }
+        if (single) for (auto& v : Lx) v = (double)(float)v;
     }
     template <class T>
     void solve(const T* b, T* xout) const {
+        if (single) { solve_single(b, xout); return; }
         std::vector<double> y(n);
         for (int k = 0; k < n; ++k) y[k] = (double)b[perm[k]];
         for (int j = 0; j < n; ++j) {
@@ -183,6 +193,22 @@ struct Cholesky {
             double s = y[j];
             for (int p = Lp[j] + 1; p < Lp[j + 1]; ++p) s -= Lx[p] * y[Li[p]];
             y[j] = s / Lx[Lp[j]];
+        }
+        for (int k = 0; k < n; ++k) xout[perm[k]] = (T)y[k];
+    }
+    // single-precision substitution with the float-rounded factor
+    template <class T>
+    void solve_single(const T* b, T* xout) const {
+        std::vector<float> y(n);
+        for (int k = 0; k < n; ++k) y[k] = (float)b[perm[k]];
+        for (int j = 0; j < n; ++j) {
+            y[j] /= (float)Lx[Lp[j]];
+            for (int p = Lp[j] + 1; p < Lp[j + 1]; ++p) y[Li[p]] -= (float)Lx[p] * y[j];
+        }
+        for (int j = n - 1; j >= 0; --j) {
+            float s = y[j];
+            for (int p = Lp[j] + 1; p < Lp[j + 1]; ++p) s -= (float)Lx[p] * y[Li[p]];
+            y[j] = s / (float)Lx[Lp[j]];
         }
         for (int k = 0; k < n; ++k) xout[perm[k]] = (T)y[k];
     }
@@ -197,15 +223,18 @@ struct SigmaMG {
     std::vector<std::vector<Cholesky>> Lii;    // [level][sub]
     std::vector<std::vector<T>> u, r, b;
 
-    void initialize(SigmaSetup<T>& s, int sigma_smoothing) {
+    void initialize(SigmaSetup<T>& s, int sigma_smoothing, bool single_precision = false) {
         S = &s; smoothing = sigma_smoothing;
         const int L = s.mg_levels;
         Lrr.assign(L, {}); Lii.assign(L, std::vector<Cholesky>(s.nsub));
         for (int l = 0; l < L; ++l) {
+            Lrr[l].single = single_precision;
             Lrr[l].factor(s.Arr[l], 1e-5);
 #pragma omp parallel for schedule(dynamic)
-            for (int sd = 0; sd < s.nsub; ++sd)
+            for (int sd = 0; sd < s.nsub; ++sd) {
+                Lii[l][sd].single = single_precision;
                 if (s.Aii[l][sd].rows > 0) Lii[l][sd].factor(s.Aii[l][sd], 1e-5);
+            }
         }
         u.assign(L, {}); r.assign(L, {}); b.assign(L, {});
         for (int l = 0; l < L; ++l) { u[l].assign(s.n_if[l], T(0)); r[l].assign(s.n_if[l], T(0)); b[l].assign(s.n_if[l], T(0)); }

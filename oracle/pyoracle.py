@@ -31,7 +31,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P, I, D = C.c_void_p, C.c_int, C.c_double
         L.ora_create.restype = P
-        L.ora_create.argtypes = [I, I, P, P, P, P, P, I, I, I, I, I, I, I, C.c_char_p, I]
+        L.ora_create.argtypes = [I, I, P, P, P, P, P, I, I, I, I, I, I, I, I, C.c_char_p, I]
         L.ora_destroy.argtypes = [P]
         L.ora_setup_seconds.restype = D
         L.ora_setup_seconds.argtypes = [P]
@@ -55,6 +55,13 @@ def lib():
         L.ora_matrix_data.argtypes = [P, I, I, I, P, P, P]
         L.ora_sigma_vcycle.argtypes = [P, P, P]
         L.ora_sigma_smooth.argtypes = [P, I, P, P]
+        L.ora_set_threads.argtypes = [I]
+        L.ora_max_threads.restype = I
+        L.ora_adaptive_stride.argtypes = [I, I, P, P]
+        L.ora_level_get.argtypes = [P, I, I, P]
+        L.ora_level_set.argtypes = [P, I, I, P]
+        L.ora_level_kernel.argtypes = [P, I, I, I]
+        L.ora_sigma_factor_solve.argtypes = [P, I, I, I, P, P]
         L.ora_solve.restype = I
         L.ora_solve.argtypes = [P, D, I, P, I, P]
         _lib = L
@@ -68,10 +75,29 @@ def _ptr(a):
 CH_Z, CH_R, CH_K, CH_X, CH_B, CH_Q, CH_S = range(7)
 
 
-class Oracle:
-    """Full reference setup + solver on one SPGrid domain (float or double)."""
+def set_threads(n: int) -> int:
+    """OpenMP threads of later oracle calls; returns the previous setting."""
+    prev = lib().ora_max_threads()
+    lib().ora_set_threads(int(n))
+    return prev
 
-    def __init__(self, dom, cfg, double: bool = False):
+
+def adaptive_stride(lo: int, hi: int):
+    """STRIDE_GENERATOR::Generate_Adaptive_Stride on one axis: (strides over [lo, hi], level)."""
+    q = np.zeros(hi - lo + 1, dtype=np.int32)
+    lev = C.c_int()
+    lib().ora_adaptive_stride(lo, hi, _ptr(q), C.byref(lev))
+    return q, lev.value
+
+
+class Oracle:
+    """Full reference setup + solver on one SPGrid domain (float or double).
+
+    sigma_single: the interface solver's exact solves in single precision (factor
+    rounded to float, float substitution), as the reference's single-precision
+    PARDISO stores and computes them; default: exact (double) solves."""
+
+    def __init__(self, dom, cfg, double: bool = False, sigma_single: bool = False):
         L = lib()
         self.dom, self.cfg = dom, cfg
         po = np.ascontiguousarray(dom.page_offsets, dtype=np.uint64)
@@ -82,7 +108,7 @@ class Oracle:
         err = C.create_string_buffer(512)
         self.h = L.ora_create(int(double), po.shape[0], _ptr(po), _ptr(fl), _ptr(bb), _ptr(size), _ptr(sub),
                               cfg.vlevels, cfg.sigma_levels, cfg.nVcycles, cfg.interior_smoothing,
-                              cfg.boundary_smoothing, cfg.sigma_smoothing, cfg.sigma_mg, err, 512)
+                              cfg.boundary_smoothing, cfg.sigma_smoothing, cfg.sigma_mg, int(bool(sigma_single)), err, 512)
         if not self.h:
             raise RuntimeError("oracle setup failed: " + err.value.decode())
 
@@ -172,6 +198,31 @@ class Oracle:
         u = np.ascontiguousarray(u, dtype=np.float64).copy()
         lib().ora_sigma_smooth(self.h, level, _ptr(b), _ptr(u))
         return u
+
+    def level_get(self, l, c):
+        """Channel c (0 u, 1 b, 2 r) of V-cycle level l, [pages, 256] in block order."""
+        n = lib().ora_level_pages(self.h, l)
+        out = np.zeros((n, 256), dtype=np.float64)
+        lib().ora_level_get(self.h, l, c, _ptr(out))
+        return out
+
+    def level_set(self, l, c, v):
+        n = lib().ora_level_pages(self.h, l)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 256)
+        lib().ora_level_set(self.h, l, c, _ptr(v))
+
+    def level_kernel(self, l, kernel: str, mode: int = 2):
+        """One V-cycle kernel on level l: residual, correction (mode 0 interior, 1
+        boundary, 2 all), restriction (l -> l+1) or prolongation (l+1 -> l)."""
+        k = {"residual": 0, "correction": 1, "restriction": 2, "prolongation": 3}[kernel]
+        lib().ora_level_kernel(self.h, l, k, mode)
+
+    def sigma_factor_solve(self, level, which: str, sub, b):
+        """(A + 1e-5 I)^{-1} b with the oracle's factor of Arr or of subdomain sub's Aii."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        lib().ora_sigma_factor_solve(self.h, level, {"Arr": 0, "Aii": 1}[which], sub, _ptr(b), _ptr(x))
+        return x
 
     def solve(self, tol_abs: float, max_it: int = 5000):
         hist = np.zeros(max_it + 1, dtype=np.float64)

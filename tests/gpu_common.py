@@ -22,3 +22,25 @@ def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def hip_runtime():
+    """The HIP runtime the library itself links (libamdhip64 of /opt/rocm), for
+    device buffers and streams the tests own as a caller would."""
+    import ctypes as C
+    import os
+    path = "/opt/rocm/lib/libamdhip64.so"
+    H = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+    P = C.c_void_p
+    H.hipMalloc.argtypes = [C.POINTER(P), C.c_size_t]
+    H.hipFree.argtypes = [P]
+    H.hipMemcpy.argtypes = [P, P, C.c_size_t, C.c_int]
+    H.hipStreamCreate.argtypes = [C.POINTER(P)]
+    H.hipStreamDestroy.argtypes = [P]
+    H.hipStreamSynchronize.argtypes = [P]
+    for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipStreamCreate", "hipStreamDestroy", "hipStreamSynchronize"):
+        getattr(H, f).restype = C.c_int
+    return H
+
+
+HIP_H2D, HIP_D2H = 1, 2

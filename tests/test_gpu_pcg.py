@@ -1,10 +1,22 @@
 """End-to-end DD-PCG on the GPU against the oracle's CPU dd_pcg.
 
-Bar: iteration count within +-1 of the oracle at the same stopping rule
-(max|r| <= 1e-6 max|b|), residual history (max|r| per iteration) within 1e-4
-relative of the oracle's over the common prefix (measured 1.4e-6 to 2.2e-5:
-the GPU's interface factors are stored in float, the oracle's solves run in
-double), and the final max|r| below tolerance.
+Two modes of the interface solver:
+
+* exact-solve parity (one step of iterative refinement after every sigma
+  solve): the GPU's exact solves then agree with the oracle's double-precision
+  ones to float rounding, and the bar is the one for the whole path: iteration
+  count within +-1 of the oracle's and the residual history (max|r| per
+  iteration) within 1e-5 relative over the common prefix;
+* the default mode (FP32 factors, one solve each, the precision of the
+  reference's single-precision PARDISO): iteration count within +-1 and the
+  history within 5e-4. Single-precision sigma solves move the history by up
+  to ~1.3e-4 from exact ones on the 64^3 workload whatever implements them
+  (tests/test_oracle_pinning.py::test_single_precision_sigma_solves_move_histories
+  measures it on the oracle itself), so 1e-5 is not a bar any single-precision
+  solver meets.
+
+Both stop at max|r| <= 1e-6 max|b|. Cases include the 64^3 workload itself and
+the 256^3 workload's DD parameters (sub_size 32, vlevels 4) at 64^3 and 96^3.
 """
 import numpy as np
 import pytest
@@ -14,12 +26,32 @@ from tests.gpu_common import random_rhs, rel_err
 
 pytestmark = pytest.mark.gpu
 
+
+def _refgen_via_disk():
+    """The reference 3D generator's domain at 64^3, written to and read back from
+    the reference's SPGrid directory format (what its dd_pcg reads)."""
+    import tempfile
+    from hierarchicaldd_amd import spgrid_io
+    dom, cfg = domain.reference_domain_3d(64, delta=(16, 16, 16))
+    with tempfile.TemporaryDirectory() as d:
+        spgrid_io.write_domain(d, dom)
+        back = spgrid_io.read_domain(d, dom.size, meta=dom.meta)
+    return back, cfg
+
+
 CASES = {
     "box32_sub16": lambda: domain.small_box(32, 16, 3, seed=1),
+    "refgen64_sub16": _refgen_via_disk,
     "sphere64_sub16": domain.config0,
     # the sparse-spheres geometry (zero Dirichlet outside a union of spheres, N(0,1) rhs) at 96^3
     "sparse96_sub16": lambda: domain.sparse_spheres(96, 6, 6, 18, seed=2, sub=16, vlevels=3),
+    # sub_size 32 (adaptive strides up to 8), vlevels 4: the 256^3 workload's parameters
+    "box64_sub32": lambda: domain.small_box(64, 32, 4, seed=3),
+    "sphere96_sub32": lambda: domain.sphere_case(96, 32, 4, delta=(32, 32, 32)),
 }
+
+HISTORY_BAR_EXACT = 1e-5
+HISTORY_BAR_SINGLE = 5e-4
 
 
 @pytest.fixture(scope="module", params=list(CASES))
@@ -33,13 +65,27 @@ def pair(request, oracle_lib):
     g.close()
 
 
+def _history_rel(h_g, h_o):
+    n = min(len(h_o), len(h_g))
+    return float((np.abs(h_g[:n] - h_o[:n]) / np.maximum(h_o[:n], 1e-30)).max())
+
+
 def test_apply_preconditioner(pair):
-    _, dom, cfg, o, g = pair
+    name, dom, cfg, o, g = pair
     r = random_rhs(g.level(0)[1], 7)
     zo = o.apply_preconditioner(r)
     zg = g.apply_preconditioner(r)
     assert np.isfinite(zg).all()
-    assert rel_err(zg, zo) < 2e-4, rel_err(zg, zo)
+    e1 = rel_err(zg, zo)
+    g.set_sigma_refinement(1)
+    try:
+        zr = g.apply_preconditioner(r)
+    finally:
+        g.set_sigma_refinement(0)
+    e2 = rel_err(zr, zo)
+    print(f"[{name}] preconditioner rel err: single {e1:.2e}, refined {e2:.2e}")
+    assert e1 < 2e-4, e1
+    assert e2 < 1e-5, e2
 
 
 def test_pcg_matches_oracle(pair):
@@ -47,12 +93,19 @@ def test_pcg_matches_oracle(pair):
     tol = 1e-6 * float(np.abs(dom.b).max())
     it_o, h_o, _ = o.solve(tol, 200)
     it_g, h_g, secs = g.solve(tol, 200)
-    print(f"[{name}] oracle iters {it_o} gpu iters {it_g} gpu seconds {secs:.4f}")
-    n = min(len(h_o), len(h_g))
-    rel = np.abs(h_g[:n] - h_o[:n]) / np.maximum(h_o[:n], 1e-30)
-    print(f"[{name}] residual-history max rel diff {rel.max():.3e}")
+    rel = _history_rel(h_g, h_o)
+    print(f"[{name}] oracle iters {it_o} gpu iters {it_g} gpu seconds {secs:.4f} history max rel diff {rel:.3e}")
     assert it_o > 0 and it_g > 0
     assert abs(it_g - it_o) <= 1
     assert h_g[-1] <= tol
-    # history agreement over the common prefix
-    assert rel.max() < 1e-4
+    assert rel < HISTORY_BAR_SINGLE
+    g.set_sigma_refinement(1)
+    try:
+        it_r, h_r, secs_r = g.solve(tol, 200)
+    finally:
+        g.set_sigma_refinement(0)
+    rel_r = _history_rel(h_r, h_o)
+    print(f"[{name}] refined: gpu iters {it_r} seconds {secs_r:.4f} history max rel diff {rel_r:.3e}")
+    assert it_r > 0 and abs(it_r - it_o) <= 1
+    assert h_r[-1] <= tol
+    assert rel_r < HISTORY_BAR_EXACT

@@ -10,6 +10,9 @@ CASES = {
     "box48_sub16_v2": lambda: domain.small_box(48, 16, 2, seed=2),
     "box24_sub8_sig2": lambda: (domain.small_box(24, 8, 2, seed=4)[0], domain.DDConfig(sub_size=8, vlevels=2, sigma_levels=2)),
     "sphere64_sub16": domain.config0,
+    # the DD parameters of the 256^3 workload (sub_size 32: adaptive strides up to 8; vlevels 4)
+    "box64_sub32_v4": lambda: domain.small_box(64, 32, 4, seed=3),
+    "sphere96_sub32_v4": lambda: domain.sphere_case(96, 32, 4, delta=(32, 32, 32)),
 }
 
 

@@ -116,3 +116,43 @@ def test_padded_size_rule():
     assert spgrid_io.padded_size_3d(256, 256, 256) == (256, 256, 256)
     assert spgrid_io.padded_size_3d(100, 100, 100) == (128, 128, 128)
     assert spgrid_io.padded_size_3d(16, 16, 128) == (64, 64, 128)
+
+
+def test_3d_generator_slice_matches_2d_fixture(fixture_dir):
+    """The 3D generator's rules (domain.reference_domain_3d, GEN/Create_3D_Domain/
+    main.cpp) at 256^3: the slice through the bubble's centre (z = 127, where the
+    cell centre (127 - 0.5)/253 is 0.5) has the 2D generator's Active / Dirichlet /
+    empty pattern, which the reference's own 2D output pins; the delta carries 4.0."""
+    size, off, fl, data = spgrid_io.read_spgrid(fixture_dir)
+    n, B = 256, 16
+    bx = ((off >> np.uint64(12)) >> np.uint64(1)).astype(np.int64)
+    by = (off >> np.uint64(12)).astype(np.int64)
+    ox = np.zeros_like(bx); oy = np.zeros_like(by)
+    for k in range(16):
+        ox |= ((bx >> (2 * k)) & 1) << k
+        oy |= ((by >> (2 * k)) & 1) << k
+    fix = np.zeros((n, n), dtype=np.uint32)
+    for i in range(len(off)):
+        fix[ox[i] * B:(ox[i] + 1) * B, oy[i] * B:(oy[i] + 1) * B] = fl[i].reshape(B, B)
+    kind2 = fix & (domain.ACTIVE | domain.DIRICHLET)
+    dom, cfg = domain.reference_domain_3d(256, delta=(64, 64, 64))
+    flags3 = dom.dense(dom.flags)
+    kind3 = flags3[:n, :n, 127] & (domain.ACTIVE | domain.DIRICHLET)
+    assert np.array_equal(kind3, kind2)
+    b3 = dom.dense(dom.b)
+    assert b3[64, 64, 64] == np.float32(4.0) and np.count_nonzero(b3) == 1
+
+
+def test_3d_generator_roundtrip_and_rules(tmp_path):
+    dom, cfg = domain.reference_domain_3d(64, delta=(16, 16, 16))
+    f = dom.dense(dom.flags)[:64, :64, :64]
+    act = (f & domain.ACTIVE) != 0
+    # the bubble: radius 0.3 of the unit box around its centre, Neumann (flag 0) cells
+    dX = np.float32(1) / np.float32(61)
+    c = (np.arange(64).astype(np.float32) - np.float32(0.5)) * dX - np.float32(0.5)
+    inside = (c[:, None, None] ** 2 + c[None, :, None] ** 2 + c[None, None, :] ** 2) <= np.float32(0.09)
+    assert not (act & inside).any()
+    assert act[1:63, 1:63, 1:63].sum() == (~inside[1:63, 1:63, 1:63]).sum()
+    spgrid_io.write_domain(str(tmp_path), dom)
+    back = spgrid_io.read_domain(str(tmp_path), dom.size)
+    assert np.array_equal(back.flags, dom.flags) and np.array_equal(back.page_offsets, dom.page_offsets)
