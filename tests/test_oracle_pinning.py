@@ -263,3 +263,49 @@ def test_single_precision_sigma_solves_move_histories(oracle_lib):
     print(f"single vs exact sigma solves: history max rel diff {rel.max():.3e}, normalised by max|r0| {rel0.max():.3e}")
     assert rel.max() > 1e-2
     assert rel0.max() < 1e-6
+
+
+# ---- interface prolongation Prr: the trilinear node rule of
+# PROLONGATION_MATRIX_HELPER::Construct_Prolongation_Matrix (IFS/PROLONGATION_MATRIX_HELPER.cpp:13-40),
+# restated on the oracle's interface node lattices of consecutive sigma levels
+
+def _prr_by_rule(fine_xyz, coarse_xyz):
+    """P[f, c] = prod_v (2 - |2 c_v - f_v|) / 2 over the coarse nodes c in
+    floor(f/2) + {0, 1}^3 that exist at the coarser level (others dropped)."""
+    at = {tuple(map(int, c)): k for k, c in enumerate(coarse_xyz)}
+    rows, cols, vals = [], [], []
+    for i, f in enumerate(fine_xyz):
+        base = [int(v) // 2 for v in f]
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    c = (base[0] + dx, base[1] + dy, base[2] + dz)
+                    w = 1.0
+                    for cv, fv in zip(c, f):
+                        w *= (2 - abs(2 * cv - int(fv))) / 2.0
+                    k = at.get(c)
+                    if w > 0 and k is not None:
+                        rows.append(i); cols.append(k); vals.append(w)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(fine_xyz), len(coarse_xyz)))
+
+
+@pytest.mark.parametrize("case", ["box32_sub16", "box64_sub32"])
+def test_interface_prolongation_is_the_trilinear_node_rule(case, oracle_lib):
+    if case == "box32_sub16":
+        dom, cfg = domain.small_box(32, 16, 3, seed=1)
+    else:
+        dom, cfg = domain.small_box(64, 32, 4, seed=3)
+    o = oracle_lib.Oracle(dom, cfg)
+    checked = 0
+    for l in range(o.sigma_levels - 1):
+        r, c, ptr, idx, val = o.matrix("Prr", l)
+        P = sp.csr_matrix((val, idx, ptr), shape=(r, c)).toarray()
+        expect = _prr_by_rule(o.interface_coords(l), o.interface_coords(l + 1)).toarray()
+        assert P.shape == expect.shape
+        assert np.array_equal(P, expect)
+        # rows whose coarse neighbours all exist sum to exactly 1 (the reference asserts total_weight == 1)
+        full = expect.sum(axis=1) == 1.0
+        assert full.mean() > 0.5
+        assert np.all(P[full].sum(axis=1) == 1.0)
+        checked += r
+    assert checked > 0
