@@ -27,10 +27,12 @@ The other two parts of the metric are added to the same line:
       difference from its solve alone;
   "served": a request stream (edge log-uniform 32-384, up to 40% blocked by
       spheres, N(0,1) right-hand side, tolerance 1e-6) with Poisson arrivals at
-      the rates of --serve-rates, through the serving loop (host setup workers,
-      groups of batched solves side by side on the GPU); p50/p99 latency
-      (submit to result, host setup included) and solves/s per rate, beside the
-      plain path (each request set up and solved alone, one after another).
+      the rates of --serve-rates, through the serving loop (host setup workers;
+      continuous batching: units of the requests ready together iterate side by
+      side on the GPU, and a unit admitted while others are mid-solve joins them
+      at their next iteration); p50/p99 latency (submit to result, host setup
+      included) and solves/s per rate, beside the plain path (each request set
+      up and solved alone, one after another).
 """
 from __future__ import annotations
 
@@ -431,7 +433,7 @@ def served_leg(hdd, n_req, rates, nmax, seed=1):
            "plain_p50_s": float(np.percentile(plain_lat, 50)), "rates": []}
     for lam in rates:
         arr = np.cumsum(rng.exponential(1.0 / lam, size=n_req))
-        srv = hdd.Server(setup_threads=4, groups=2, max_batch=16)
+        srv = hdd.Server(setup_threads=4, groups=4, max_batch=16)
         t0 = time.perf_counter()
         ids = []
         for (dom, cfg), tol, ta in zip(reqs, tols, arr):
@@ -442,7 +444,8 @@ def served_leg(hdd, n_req, rates, nmax, seed=1):
         reps = []
         for k, i in enumerate(ids):
             reps.append(srv.wait(i)[2])
-            log(f"[served] rate {lam}/s request {k}: latency {reps[-1]['latency_s']:.2f}s (group of {reps[-1]['batch_size']})")
+            log(f"[served] rate {lam}/s request {k}: latency {reps[-1]['latency_s']:.2f}s (unit of {reps[-1]['batch_size']}, "
+                f"{reps[-1]['concurrent']} units on the device when it started)")
         span = time.perf_counter() - t0
         srv.close()
         lat = [r["latency_s"] for r in reps]
@@ -450,7 +453,9 @@ def served_leg(hdd, n_req, rates, nmax, seed=1):
         row = {"rate_per_s": lam, "solves_per_s": n_req / span, "p50_latency_s": float(np.percentile(lat, 50)),
                "p99_latency_s": float(np.percentile(lat, 99)), "mean_setup_s": float(np.mean([r["setup_s"] for r in reps])),
                "mean_queue_s": float(np.mean([r["queue_s"] for r in reps])), "mean_solve_s": float(np.mean([r["solve_s"] for r in reps])),
-               "max_group": int(max(r["batch_size"] for r in reps)), "all_converged": ok}
+               "max_group": int(max(r["batch_size"] for r in reps)),
+               "max_concurrent_units": int(max(r["concurrent"] for r in reps)),
+               "joined_mid_solve": int(sum(r["concurrent"] >= 2 for r in reps)), "all_converged": ok}
         log(f"[served] rate {lam}/s: {row}")
         out["rates"].append(row)
     return out

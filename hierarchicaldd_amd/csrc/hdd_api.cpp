@@ -147,9 +147,12 @@ hdd_status hdd_batch_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_
         bool all = true;
         for (int p = 0; p < np; ++p) {
             const PcgResult& r = res[p];
-            if (rep) { rep[p].iterations = r.iterations; rep[p].seconds = r.seconds; rep[p].final_norm = r.hist.empty() ? 0.0 : r.hist.back(); }
+            if (rep) {
+                rep[p].iterations = r.iterations; rep[p].seconds = r.seconds;
+                rep[p].final_norm = r.hist.empty() ? 0.0 : r.hist.back(); rep[p].converged = r.converged;
+            }
             if (hist) for (int i = 0; i < hist_len; ++i) hist[(size_t)p * hist_len + i] = i < (int)r.hist.size() ? r.hist[i] : 0.0;
-            all = all && r.iterations >= 0;
+            all = all && r.converged;
         }
         return all ? HDD_OK : HDD_ERR_NOT_CONVERGED;
     } catch (std::exception& e) {
@@ -406,10 +409,11 @@ hdd_status hdd_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_report
     try {
         std::vector<double> h;
         double secs = 0;
-        int it = ctx->solver->pcg(prm->tolerance, prm->max_iterations, h, &secs);
-        if (rep) { rep->iterations = it; rep->seconds = secs; rep->final_norm = h.empty() ? 0.0 : h.back(); }
+        bool conv = false;
+        int it = ctx->solver->pcg(prm->tolerance, prm->max_iterations, h, &secs, &conv);
+        if (rep) { rep->iterations = it; rep->seconds = secs; rep->final_norm = h.empty() ? 0.0 : h.back(); rep->converged = conv; }
         if (hist) for (int i = 0; i < (int)h.size() && i < hist_len; ++i) hist[i] = h[i];
-        return it < 0 ? HDD_ERR_NOT_CONVERGED : HDD_OK;
+        return conv ? HDD_OK : HDD_ERR_NOT_CONVERGED;
     } catch (std::exception& e) {
         ctx->err = e.what();
         return HDD_ERR_DEVICE;
@@ -512,6 +516,8 @@ hdd_status hdd_profile_solve(hdd_ctx* ctx, int level, int which, int reps, doubl
     })
 }
 
+static constexpr int kPlanStatsPerHeight = 8;
+
 static void plan_stats_impl(const SigmaHost& H, int level, int which, double* out, int cap, int* heights) {
     if (level < 0 || level >= H.levels) throw std::runtime_error("no such sigma level");
     std::vector<SupernodalFactor> f;
@@ -538,14 +544,21 @@ static void plan_stats_impl(const SigmaHost& H, int level, int which, double* ou
     TriPlanHost P;
     build_triplan(fp, fo, P);
     *heights = P.H + 1;
-    std::vector<double> v(5 * (size_t)(P.H + 1), 0.0);
+    constexpr int K = kPlanStatsPerHeight;
+    std::vector<double> v(K * (size_t)(P.H + 1), 0.0);
     for (int h = 0; h <= P.H; ++h) {
         for (int q = P.sn_begin[h]; q < P.sn_begin[h + 1]; ++q) {
             const int g = P.sn_list[q];
-            v[5 * h + 0] += 1; v[5 * h + 1] += P.sz[g];
-            v[5 * h + 2] += 0.5 * P.sz[g] * (P.sz[g] + 1.0); v[5 * h + 3] += (double)P.sz[g] * P.nr[g];
+            v[K * h + 0] += 1; v[K * h + 1] += P.sz[g];
+            v[K * h + 2] += 0.5 * P.sz[g] * (P.sz[g] + 1.0); v[K * h + 3] += (double)P.sz[g] * P.nr[g];
+            v[K * h + 5] = std::max(v[K * h + 5], (double)P.nr[g]);
+            for (int i = P.s0[g]; i < P.s0[g] + P.sz[g]; ++i) {
+                const int inc = P.inc_ptr[i + 1] - P.inc_ptr[i];
+                v[K * h + 6] = std::max(v[K * h + 6], (double)inc);
+                v[K * h + 7] += inc;
+            }
         }
-        v[5 * h + 4] = P.max_sz[h];
+        v[K * h + 4] = P.max_sz[h];
     }
     for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
 }
@@ -593,7 +606,7 @@ hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas,
 
 struct hdd_server {
     std::unique_ptr<Server> srv;
-    std::mutex mu;
+    mutable std::mutex mu;
     std::string err;
 };
 
@@ -636,26 +649,41 @@ hdd_status hdd_server_wait(hdd_server* s, int64_t id, float* x, hdd_request_repo
     try {
         std::shared_ptr<ServerRequest> r = s->srv->wait(id);
         s->srv->forget(id);
+        const bool ran = r->t_admit > 0;
         if (rep) {
             rep->iterations = r->iterations; rep->final_norm = r->final_norm;
             rep->latency_s = r->t_done - r->t_submit;
-            rep->setup_s = r->t_ready - r->t_submit;
-            rep->queue_s = r->failed ? 0.0 : r->t_admit - r->t_ready;
-            rep->solve_s = r->failed ? 0.0 : r->t_done - r->t_admit;
+            rep->setup_s = r->t_ready > 0 ? r->t_ready - r->t_submit : 0.0;
+            rep->queue_s = ran ? r->t_admit - r->t_ready : 0.0;
+            rep->solve_s = ran ? r->t_done - r->t_admit : 0.0;
             rep->batch_size = r->batch_size;
+            rep->concurrent = r->concurrent;
+            rep->cancelled = r->cancelled;
         }
         if (r->failed) {
             std::lock_guard<std::mutex> lk(s->mu);
             s->err = r->error;
-            return r->t_admit > 0 ? HDD_ERR_DEVICE : HDD_ERR_SETUP;
+            return ran ? HDD_ERR_DEVICE : HDD_ERR_SETUP;
         }
-        if (x) std::memcpy(x, r->x.data(), r->x.size() * sizeof(float));
-        return r->iterations < 0 ? HDD_ERR_NOT_CONVERGED : HDD_OK;
+        if (x) {
+            if (r->x.empty()) std::memset(x, 0, r->offsets.size() * kPageCells * sizeof(float));
+            else std::memcpy(x, r->x.data(), r->x.size() * sizeof(float));
+        }
+        if (r->cancelled) return HDD_ERR_CANCELLED;
+        return r->converged ? HDD_OK : HDD_ERR_NOT_CONVERGED;
     } catch (std::exception& e) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = e.what();
         return HDD_ERR_ARG;
     }
+}
+
+hdd_status hdd_server_cancel(hdd_server* s, int64_t id) {
+    if (!s) return HDD_ERR_ARG;
+    if (s->srv->cancel(id)) return HDD_OK;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->err = "hdd_server_cancel: no such request, or it is already done";
+    return HDD_ERR_ARG;
 }
 
 hdd_status hdd_server_destroy(hdd_server* s) {
@@ -664,9 +692,16 @@ hdd_status hdd_server_destroy(hdd_server* s) {
     return HDD_OK;
 }
 
+// The server's message is shared by the threads that submit and wait; the
+// caller gets a copy taken under the lock, kept per calling thread.
 const char* hdd_server_last_error(const hdd_server* s) {
     if (!s) return g_create_error.c_str();
-    return s->err.c_str();
+    static thread_local std::string copy;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        copy = s->err;
+    }
+    return copy.c_str();
 }
 
 hdd_status hdd_server_budget(const hdd_server* s, double* bytes) {

@@ -604,26 +604,50 @@ void Solver::interface_scatter() {
     launch_interface_scatter(*sigma, if_index.p, L0.u[L0.cur].p, stream);
 }
 
+// While a batch PCG runs, the stencil kernels and exact solves skip the
+// problems that have stopped (their pages and supernodes are left untouched).
+namespace {
+struct SkipStopped {
+    Solver& S;
+    explicit SkipStopped(Solver& s) : S(s) { set(S.active.p); }
+    ~SkipStopped() {
+        set(nullptr);
+        std::vector<int> ones(S.nprob(), 1);
+        (void)hipMemcpyAsync(S.active.p, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice, S.stream);
+        (void)hipStreamSynchronize(S.stream);
+    }
+    void set(const int* a) {
+        for (auto& L : S.lv) L.active = a;
+        if (S.sigma) S.sigma->set_active(a);
+    }
+};
+}  // namespace
+
 // CONJUGATE_GRADIENT_DUMPING::Solve (DDS/CONJUGATE_GRADIENT_DUMPING.cpp:146-212)
 // for every problem of the batch, x starting at zero, with the stopping rule
 // max|r| <= tol[p]. Each problem has its own scalars; once it stops, its x
-// and r are frozen (the shared launches still visit its pages).
-void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out,
-                       const std::function<void(int)>& on_stop) {
+// and r are frozen (the shared launches still visit its pages). The loop is
+// split at its one host synchronisation per iteration (the residual test):
+// pcg_start queues the set-up and the first test, pcg_advance waits for a
+// test, records it and queues the next iteration.
+void Solver::pcg_start(PcgRun& R, const std::vector<double>& tol, const std::vector<int>& max_it) {
     const int NP = nprob();
     if ((int)tol.size() != NP || (int)max_it.size() != NP) throw std::runtime_error("pcg: one tolerance and iteration limit per problem");
     LevelBuffers& L0 = lv[0];
     const DevLevel v0 = L0.view();
     const size_t n0 = L0.cells();
-    float* r = L0.b.p;
-    const DevSegments sg = segments();
-    out.assign(NP, PcgResult());
+    R.tol = tol; R.max_it = max_it;
+    R.out.assign(NP, PcgResult());
+    R.act.assign(NP, 1);
+    R.cancel.assign(NP, 0);
+    R.it = 0; R.n_active = NP; R.started = true;
     const float flt_max = 3.40282347e+38f;
-    std::vector<float> hs((size_t)NP * 8, 0.f);
-    for (int p = 0; p < NP; ++p) hs[(size_t)p * 8 + 3] = flt_max;  // rho_old
-    std::vector<int> act(NP, 1);
-    const bool fp64 = outer_fp64;
-    if (fp64) {
+    hs_pin.alloc((size_t)NP * 8);
+    act_pin.alloc((size_t)NP);
+    for (size_t i = 0; i < (size_t)NP * 8; ++i) hs_pin.p[i] = 0.f;
+    for (int p = 0; p < NP; ++p) { hs_pin.p[(size_t)p * 8 + 3] = flt_max; act_pin.p[p] = 1; }  // rho_old
+    R.fp64 = outer_fp64;
+    if (R.fp64) {
         if (xd.n != n0) xd.alloc(n0);
         std::vector<double> hd((size_t)NP * 8, 0.0);
         for (int p = 0; p < NP; ++p) hd[(size_t)p * 8 + 5] = flt_max;  // rho_old
@@ -631,85 +655,90 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
         hdd_check(hipMemsetAsync(xd.p, 0, n0 * sizeof(double), stream));
         hdd_check(hipStreamSynchronize(stream));  // hd goes out of scope
     }
-    hdd_check(hipMemcpyAsync(scal.p, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-    hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
-    // while the batch iterates, the stencil kernels and exact solves skip the problems that have stopped
-    struct SkipStopped {
-        Solver& S;
-        explicit SkipStopped(Solver& s) : S(s) { set(S.active.p); }
-        ~SkipStopped() {
-            set(nullptr);
-            std::vector<int> ones(S.nprob(), 1);
-            (void)hipMemcpyAsync(S.active.p, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice, S.stream);
-            (void)hipStreamSynchronize(S.stream);
-        }
-        void set(const int* a) {
-            for (auto& L : S.lv) L.active = a;
-            if (S.sigma) S.sigma->set_active(a);
-        }
-    };
-    std::unique_ptr<SkipStopped> skip(NP > 1 ? new SkipStopped(*this) : nullptr);
+    hdd_check(hipMemcpyAsync(scal.p, hs_pin.p, (size_t)NP * 8 * sizeof(float), hipMemcpyHostToDevice, stream));
+    hdd_check(hipMemcpyAsync(active.p, act_pin.p, NP * sizeof(int), hipMemcpyHostToDevice, stream));
+    R.skip.reset();
+    if (NP > 1) R.skip = std::shared_ptr<void>(new SkipStopped(*this), [](void* q) { delete static_cast<SkipStopped*>(q); });
     launch_fill(x.p, 0.f, n0, stream);
     hdd_check(hipStreamSynchronize(stream));
-    auto t0 = std::chrono::steady_clock::now();
-    int n_active = NP;
-    for (int it = 0;; ++it) {
-        const bool restart = (it == 0);
-        if (restart) {
-            launch_laplace_pcg(v0, x.p, q.p, stream);
-            launch_sub_project(v0, bvec.p, q.p, r, stream);
+    R.t0 = std::chrono::steady_clock::now();
+    // iteration 0 (a restart): r = Project(b - L x), then its test
+    launch_laplace_pcg(v0, x.p, q.p, stream);
+    launch_sub_project(v0, bvec.p, q.p, L0.b.p, stream);
+    launch_seg_maxabs(segments(), L0.b.p, stream);
+    hdd_check(hipMemcpyAsync(hs_pin.p, scal.p, (size_t)NP * 8 * sizeof(float), hipMemcpyDeviceToHost, stream));
+}
+
+bool Solver::pcg_advance(PcgRun& R, const std::function<void(int)>& on_stop) {
+    if (!R.started) throw std::runtime_error("pcg_advance before pcg_start");
+    if (R.n_active == 0) return true;
+    const int NP = nprob();
+    LevelBuffers& L0 = lv[0];
+    DevLevel v0 = L0.view();
+    v0.active = nullptr;  // the PCG operator visits every page; only the PCG vector updates skip stopped problems
+    const size_t n0 = L0.cells();
+    float* r = L0.b.p;
+    const DevSegments sg = segments();
+    float* hs = hs_pin.p;
+    const int it = R.it;
+    const bool restart = (it == 0);
+    hdd_check(hipStreamSynchronize(stream));
+    if (R.fp64 && it > 0) {
+        // residual replacement: every replace_every iterations, and whenever a problem's
+        // recursive residual passes its test, r becomes the true residual b - L x of the
+        // double iterate, and the test is taken on that
+        bool replace = replace_every > 0 && it % replace_every == 0;
+        for (int p = 0; p < NP && !replace; ++p) replace = R.act[p] && hs[(size_t)p * 8 + 4] <= (float)R.tol[p];
+        if (replace) {
+            launch_true_residual(v0, xd.p, bvec.p, r, stream);
+            launch_seg_maxabs(sg, r, stream);
+            hdd_check(hipMemcpyAsync(hs, scal.p, (size_t)NP * 8 * sizeof(float), hipMemcpyDeviceToHost, stream));
+            hdd_check(hipStreamSynchronize(stream));
         }
-        launch_seg_maxabs(sg, r, stream);
-        hdd_check(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+    }
+    const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - R.t0).count();
+    bool changed = false;
+    for (int p = 0; p < NP; ++p) {
+        if (!R.act[p]) continue;
+        const float conv = hs[(size_t)p * 8 + 4];
+        PcgResult& o = R.out[p];
+        o.hist.push_back((double)conv);
+        if (conv <= (float)R.tol[p]) { o.iterations = it; o.converged = true; }
+        else if (R.cancel[p]) { o.iterations = -it; o.cancelled = true; }
+        else if (it >= R.max_it[p]) o.iterations = -it;
+        else continue;
+        o.seconds = now;
+        R.act[p] = 0;
+        --R.n_active;
+        changed = true;
+        if (R.fp64) {
+            // the float x channel of the stopped problem from its double iterate
+            const size_t first = (size_t)pg0[0][p] * kPageCells, cnt = (size_t)(pg0[0][p + 1] - pg0[0][p]) * kPageCells;
+            launch_cast_to_float(xd.p + first, x.p + first, cnt, stream);
+            hdd_check(hipStreamSynchronize(stream));
+        }
+        if (on_stop) on_stop(p);
+    }
+    if (R.n_active == 0) {
         hdd_check(hipStreamSynchronize(stream));
-        if (fp64 && it > 0) {
-            // residual replacement: every replace_every iterations, and whenever a problem's
-            // recursive residual passes its test, r becomes the true residual b - L x of the
-            // double iterate, and the test is taken on that
-            bool replace = replace_every > 0 && it % replace_every == 0;
-            for (int p = 0; p < NP && !replace; ++p) replace = act[p] && hs[(size_t)p * 8 + 4] <= (float)tol[p];
-            if (replace) {
-                launch_true_residual(v0, xd.p, bvec.p, r, stream);
-                launch_seg_maxabs(sg, r, stream);
-                hdd_check(hipMemcpyAsync(hs.data(), scal.p, hs.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
-                hdd_check(hipStreamSynchronize(stream));
-            }
-        }
-        const double now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        bool changed = false;
-        for (int p = 0; p < NP; ++p) {
-            if (!act[p]) continue;
-            const float conv = hs[(size_t)p * 8 + 4];
-            out[p].hist.push_back((double)conv);
-            if (conv <= (float)tol[p]) out[p].iterations = it;
-            else if (it >= max_it[p]) out[p].iterations = -it;
-            else continue;
-            out[p].seconds = now;
-            act[p] = 0;
-            --n_active;
-            changed = true;
-            if (fp64) {
-                // the float x channel of the stopped problem from its double iterate
-                const size_t first = (size_t)pg0[0][p] * kPageCells, cnt = (size_t)(pg0[0][p + 1] - pg0[0][p]) * kPageCells;
-                launch_cast_to_float(xd.p + first, x.p + first, cnt, stream);
-                hdd_check(hipStreamSynchronize(stream));
-            }
-            if (on_stop) on_stop(p);
-        }
-        if (n_active == 0) break;
-        // act is not modified again before the next synchronisation
-        if (changed) hdd_check(hipMemcpyAsync(active.p, act.data(), NP * sizeof(int), hipMemcpyHostToDevice, stream));
-        apply_preconditioner();
-        float* z = channel(0);
-        if (fp64) {
-            launch_seg_dot(sg, z, r, 5, stream);                   // rho, beta (double)
-            if (restart) launch_copy(s.p, z, n0, stream);
-            else launch_seg_copy2_d(sg, s.p, s.p, z, 4, stream);   // s = s*beta + z
-            launch_laplace_pcg(v0, s.p, q.p, stream);              // q = Project(A s)
-            launch_seg_dot(sg, s.p, q.p, 6, stream);               // alpha, rho_old (double)
-            launch_seg_xr_update_d(sg, xd.p, r, s.p, q.p, stream); // x (double) += alpha s; r -= alpha q
-            continue;
-        }
+        R.skip.reset();
+        return true;
+    }
+    // act_pin is not modified again before the next synchronisation
+    if (changed) {
+        for (int p = 0; p < NP; ++p) act_pin.p[p] = R.act[p];
+        hdd_check(hipMemcpyAsync(active.p, act_pin.p, NP * sizeof(int), hipMemcpyHostToDevice, stream));
+    }
+    apply_preconditioner();
+    float* z = channel(0);
+    if (R.fp64) {
+        launch_seg_dot(sg, z, r, 5, stream);                   // rho, beta (double)
+        if (restart) launch_copy(s.p, z, n0, stream);
+        else launch_seg_copy2_d(sg, s.p, s.p, z, 4, stream);   // s = s*beta + z
+        launch_laplace_pcg(v0, s.p, q.p, stream);              // q = Project(A s)
+        launch_seg_dot(sg, s.p, q.p, 6, stream);               // alpha, rho_old (double)
+        launch_seg_xr_update_d(sg, xd.p, r, s.p, q.p, stream); // x (double) += alpha s; r -= alpha q
+    } else {
         launch_seg_dot(sg, z, r, 0, stream);                   // rho, beta
         if (restart) launch_copy(s.p, z, n0, stream);
         else launch_seg_copy2(sg, s.p, s.p, z, 2, stream);     // s = s*beta + z
@@ -717,13 +746,27 @@ void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& m
         launch_seg_dot(sg, s.p, q.p, 1, stream);               // alpha, rho_old
         launch_seg_xr_update(sg, x.p, r, s.p, q.p, stream);
     }
-    hdd_check(hipStreamSynchronize(stream));
+    R.it = it + 1;
+    // the next iteration's test
+    launch_seg_maxabs(sg, r, stream);
+    hdd_check(hipMemcpyAsync(hs, scal.p, (size_t)NP * 8 * sizeof(float), hipMemcpyDeviceToHost, stream));
+    return false;
 }
 
-int Solver::pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds) {
+void Solver::pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out,
+                       const std::function<void(int)>& on_stop) {
+    PcgRun R;
+    pcg_start(R, tol, max_it);
+    while (!pcg_advance(R, on_stop)) {
+    }
+    out = std::move(R.out);
+}
+
+int Solver::pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds, bool* converged) {
     std::vector<PcgResult> res;
     pcg_batch(std::vector<double>(nprob(), tol_abs), std::vector<int>(nprob(), max_it), res);
     hist = res[0].hist;
+    if (converged) *converged = res[0].converged;
     double worst = 0;
     for (auto& r : res) worst = std::max(worst, r.seconds);
     if (seconds) *seconds = worst;

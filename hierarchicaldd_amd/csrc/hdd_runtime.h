@@ -17,6 +17,7 @@
 //    problem's x and r are frozen while the others continue.
 // One problem is the special case of a batch of one.
 #pragma once
+#include <chrono>
 #include <functional>
 #include <memory>
 #include <vector>
@@ -103,8 +104,47 @@ struct LevelBuffers {
 
 struct PcgResult {
     int iterations = 0;                // negative: not converged within max_it
+    bool converged = false;            // the stopping test passed (iterations alone cannot say so at 0)
+    bool cancelled = false;            // stopped by PcgRun::cancel before its test passed
     double seconds = 0;                // from the start of the batch solve to this problem's last test
     std::vector<double> hist;          // max|r| per iteration
+};
+
+// Pinned host memory (hipHostMalloc): the PCG loop's per-iteration scalar
+// copies then run asynchronously on the stream.
+template <class T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() {}
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    void alloc(size_t count) {
+        if (count <= n) return;
+        if (p) { (void)hipHostFree(p); p = nullptr; n = 0; }
+        hdd_check(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+        n = count;
+    }
+};
+
+// A PCG solve of every problem of a Solver, advanced one iteration at a time:
+// Solver::pcg_start queues the set-up and the first residual test, and each
+// Solver::pcg_advance waits for the test, records it, stops the problems that
+// are done and queues the next iteration, so one host thread can keep several
+// Solvers' iterations in flight on their own streams (the serving loop).
+// pcg_batch is pcg_start followed by pcg_advance until every problem stops.
+struct PcgRun {
+    std::vector<double> tol;
+    std::vector<int> max_it;
+    std::vector<PcgResult> out;
+    std::vector<int> act;              // problems still iterating
+    std::vector<char> cancel;          // set by the owner between advances: stop that problem at the next test
+    int it = 0, n_active = 0;
+    bool fp64 = false, started = false;
+    std::chrono::steady_clock::time_point t0;
+    std::shared_ptr<void> skip;        // while it lives, the stencil and solve kernels skip stopped problems
+    bool done() const { return started && n_active == 0; }
 };
 
 class Solver {
@@ -147,12 +187,15 @@ public:
     void vcycle();
     void smooth0(int mode);
     void apply_preconditioner();  // z <- M^{-1} r
-    int pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds);
+    int pcg(double tol_abs, int max_it, std::vector<double>& hist, double* seconds, bool* converged = nullptr);
     // every problem from x = 0; tol[p], max_it[p] per problem. on_stop(p), when
     // given, runs on this thread as soon as problem p has stopped (its x is final
     // and the stream is idle), while the others continue.
     void pcg_batch(const std::vector<double>& tol, const std::vector<int>& max_it, std::vector<PcgResult>& out,
                    const std::function<void(int)>& on_stop = nullptr);
+    // the same loop, one iteration per call (see PcgRun); pcg_advance returns true when every problem has stopped
+    void pcg_start(PcgRun& R, const std::vector<double>& tol, const std::vector<int>& max_it);
+    bool pcg_advance(PcgRun& R, const std::function<void(int)>& on_stop = nullptr);
     // x of problem p (its level-0 pages in its own page order), n_pages(p)*256 floats
     void download_x(int p, float* host);
     // FP64 outer PCG (opt-in; the reference keeps rho, alpha, beta in float): the
@@ -184,6 +227,8 @@ public:
     void set_stream(hipStream_t s);
     hipStream_t own_stream = nullptr;
     DevSegments segments();
+    PinnedBuf<float> hs_pin;           // per-problem float scalars read back every iteration
+    PinnedBuf<int> act_pin;            // per-problem iterate flags sent to the device
 
 private:
     struct Graph {

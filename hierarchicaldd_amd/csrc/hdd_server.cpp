@@ -13,7 +13,8 @@ Server::Server(const ServerParams& p) : prm(p), t0(std::chrono::steady_clock::no
         budget_ = 0.9 * (double)free_b;
     }
     for (int i = 0; i < prm.setup_threads; ++i) threads.emplace_back([this] { setup_worker(); });
-    for (int i = 0; i < prm.groups; ++i) threads.emplace_back([this] { group_worker(); });
+    threads.emplace_back([this] { builder(); });
+    threads.emplace_back([this] { driver(); });
 }
 
 Server::~Server() {
@@ -23,6 +24,7 @@ Server::~Server() {
     }
     cv_setup.notify_all();
     cv_ready.notify_all();
+    cv_driver.notify_all();
     for (auto& t : threads) t.join();
 }
 
@@ -58,11 +60,28 @@ std::shared_ptr<ServerRequest> Server::wait(int64_t id) {
     return r;
 }
 
+bool Server::cancel(int64_t id) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = all.find(id);
+    if (it == all.end() || it->second->done) return false;
+    it->second->cancel = true;
+    return true;
+}
+
 void Server::forget(int64_t id) {
     std::lock_guard<std::mutex> lk(mu);
     auto it = all.find(id);
     if (it != all.end() && it->second->done) all.erase(it);
 }
+
+void Server::finish_cancelled(const std::shared_ptr<ServerRequest>& r) {
+    r->cancelled = true;
+    r->done = true;
+    r->t_done = now();
+    r->host.reset();
+}
+
+bool Server::drained_for_builder() const { return stopping && ready.empty() && submitted.empty() && setting_up_ == 0; }
 
 void Server::setup_worker() {
     for (;;) {
@@ -73,6 +92,13 @@ void Server::setup_worker() {
             if (submitted.empty()) return;
             r = submitted.front();
             submitted.pop_front();
+            if (r->cancel) {
+                finish_cancelled(r);
+                lk.unlock();
+                cv_done.notify_all();
+                cv_ready.notify_all();
+                continue;
+            }
             ++setting_up_;
         }
         std::string err;
@@ -108,67 +134,148 @@ static bool same_schedule(const SolverParams& a, const SolverParams& b) {
            a.sig_s == b.sig_s && a.sig_mg == b.sig_mg;
 }
 
-void Server::group_worker() {
+// Forms a unit from the ready requests and builds its device state; the driver
+// picks it up at its next round.
+void Server::builder() {
     for (;;) {
-        std::vector<std::shared_ptr<ServerRequest>> grp;
-        double bytes = 0;
+        std::unique_ptr<Unit> u(new Unit());
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_ready.wait(lk, [&] {
-                if (!ready.empty() && in_use_ + (double)ready.front()->bytes <= budget_) return true;
-                return stopping && ready.empty() && submitted.empty() && setting_up_ == 0;
-            });
-            if (ready.empty()) return;
-            // arrival order; problems with another V-cycle/sigma schedule wait for a later group
+            for (;;) {
+                // requests cancelled while they waited never reach the device
+                for (auto it = ready.begin(); it != ready.end();) {
+                    if ((*it)->cancel) { finish_cancelled(*it); it = ready.erase(it); cv_done.notify_all(); }
+                    else ++it;
+                }
+                if (!ready.empty() && units_ < prm.groups && in_use_ + (double)ready.front()->bytes <= budget_) break;
+                if (ready.empty() && drained_for_builder()) {
+                    builder_done_ = true;
+                    lk.unlock();
+                    cv_driver.notify_all();
+                    return;
+                }
+                cv_ready.wait(lk);
+            }
+            // arrival order; problems with another V-cycle/sigma schedule wait for a later unit
             const SolverParams lead = ready.front()->P;
-            for (auto it = ready.begin(); it != ready.end() && (int)grp.size() < prm.max_batch;) {
+            for (auto it = ready.begin(); it != ready.end() && (int)u->reqs.size() < prm.max_batch;) {
                 const double b = (double)(*it)->bytes;
                 if (!same_schedule((*it)->P, lead)) { ++it; continue; }
-                if (in_use_ + bytes + b > budget_) break;
-                bytes += b;
-                grp.push_back(*it);
+                if (in_use_ + u->bytes + b > budget_) break;
+                u->bytes += b;
+                u->reqs.push_back(*it);
                 it = ready.erase(it);
             }
-            in_use_ += bytes;
-            const double t = now();
-            for (auto& r : grp) { r->t_admit = t; r->batch_size = (int)grp.size(); }
+            in_use_ += u->bytes;
+            ++units_;
+            for (auto& r : u->reqs) r->batch_size = (int)u->reqs.size();
         }
         try {
             std::vector<std::shared_ptr<ProblemHost>> hosts;
-            std::vector<double> tol;
-            std::vector<int> mit;
-            for (auto& r : grp) { hosts.push_back(r->host); tol.push_back(r->tol); mit.push_back(r->max_it); }
-            Solver S(hosts);
-            std::vector<PcgResult> res;
-            S.pcg_batch(tol, mit, res, [&](int p) {
-                ServerRequest& r = *grp[p];
-                const ProblemHost& H = *hosts[p];
-                std::vector<float> xp((size_t)H.lv[0].npages() * kPageCells);
-                S.download_x(p, xp.data());
-                r.x.assign(H.input_page.size() * kPageCells, 0.f);
-                for (size_t i = 0; i < H.input_page.size(); ++i)
-                    std::copy(xp.begin() + (size_t)H.input_page[i] * kPageCells, xp.begin() + (size_t)(H.input_page[i] + 1) * kPageCells,
-                              r.x.begin() + i * kPageCells);
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    r.iterations = res[p].iterations;
-                    r.final_norm = res[p].hist.empty() ? 0.0 : res[p].hist.back();
-                    r.t_done = now();
-                    r.done = true;
-                }
-                cv_done.notify_all();
-            });
+            for (auto& r : u->reqs) hosts.push_back(r->host);
+            u->S.reset(new Solver(hosts));
         } catch (std::exception& e) {
             std::lock_guard<std::mutex> lk(mu);
-            for (auto& r : grp)
-                if (!r->done) { r->failed = true; r->error = e.what(); r->done = true; r->t_done = now(); }
+            for (auto& r : u->reqs) { r->failed = true; r->error = e.what(); r->done = true; r->t_done = now(); r->host.reset(); }
+            in_use_ -= u->bytes;
+            --units_;
+            cv_done.notify_all();
+            continue;
         }
         {
             std::lock_guard<std::mutex> lk(mu);
-            in_use_ -= bytes;
-            for (auto& r : grp) r->host.reset();
+            joining.push_back(std::move(u));
         }
-        cv_done.notify_all();
+        cv_driver.notify_all();
+    }
+}
+
+// Advances every resident unit by one PCG iteration per round.
+void Server::driver() {
+    std::vector<std::unique_ptr<Unit>> active;
+    for (;;) {
+        std::vector<std::unique_ptr<Unit>> joined;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_driver.wait(lk, [&] { return !joining.empty() || !active.empty() || (builder_done_ && joining.empty()); });
+            while (!joining.empty()) { joined.push_back(std::move(joining.front())); joining.pop_front(); }
+            if (joined.empty() && active.empty() && builder_done_) return;
+        }
+        for (auto& u : joined) {
+            std::vector<double> tol;
+            std::vector<int> mit;
+            for (auto& r : u->reqs) { tol.push_back(r->tol); mit.push_back(r->max_it); }
+            bool ok = true;
+            try {
+                u->S->pcg_start(u->run, tol, mit);
+            } catch (std::exception& e) {
+                ok = false;
+                std::lock_guard<std::mutex> lk(mu);
+                for (auto& r : u->reqs) { r->failed = true; r->error = e.what(); r->done = true; r->t_done = now(); r->host.reset(); }
+            }
+            if (!ok) {
+                u->S.reset();
+                std::lock_guard<std::mutex> lk(mu);
+                in_use_ -= u->bytes;
+                --units_;
+                cv_done.notify_all();
+                cv_ready.notify_all();
+                continue;
+            }
+            active.push_back(std::move(u));
+            std::lock_guard<std::mutex> lk(mu);
+            const double t = now();
+            for (auto& r : active.back()->reqs) { r->t_admit = t; r->concurrent = (int)active.size(); }
+        }
+        for (auto& up : active) {
+            Unit& u = *up;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (size_t p = 0; p < u.reqs.size(); ++p) u.run.cancel[p] = u.reqs[p]->cancel ? 1 : 0;
+            }
+            try {
+                u.S->pcg_advance(u.run, [&](int p) {
+                    ServerRequest& r = *u.reqs[p];
+                    const ProblemHost& H = *r.host;
+                    std::vector<float> xp((size_t)H.lv[0].npages() * kPageCells);
+                    u.S->download_x(p, xp.data());
+                    std::vector<float> xin(H.input_page.size() * kPageCells, 0.f);
+                    for (size_t i = 0; i < H.input_page.size(); ++i)
+                        std::copy(xp.begin() + (size_t)H.input_page[i] * kPageCells, xp.begin() + (size_t)(H.input_page[i] + 1) * kPageCells,
+                                  xin.begin() + i * kPageCells);
+                    const PcgResult& res = u.run.out[p];
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        r.x.swap(xin);
+                        r.iterations = res.iterations;
+                        r.converged = res.converged;
+                        r.cancelled = res.cancelled;
+                        r.final_norm = res.hist.empty() ? 0.0 : res.hist.back();
+                        r.t_done = now();
+                        r.done = true;
+                    }
+                    cv_done.notify_all();
+                });
+            } catch (std::exception& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                for (auto& r : u.reqs)
+                    if (!r->done) { r->failed = true; r->error = e.what(); r->done = true; r->t_done = now(); }
+                u.run.n_active = 0;
+                cv_done.notify_all();
+            }
+        }
+        // retire the units whose problems have all stopped: their device memory and slot go to the next unit
+        for (auto& up : active) {
+            if (up->run.n_active > 0) continue;
+            up->run.skip.reset();
+            up->S.reset();
+            std::lock_guard<std::mutex> lk(mu);
+            in_use_ -= up->bytes;
+            --units_;
+            for (auto& r : up->reqs) r->host.reset();
+            up.reset();
+        }
+        active.erase(std::remove(active.begin(), active.end(), nullptr), active.end());
         cv_ready.notify_all();
     }
 }

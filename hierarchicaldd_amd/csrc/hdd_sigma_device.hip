@@ -422,6 +422,216 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
     }
 }
 
+// Wavefront-per-run forms of the two kernels above, four runs per 256-thread
+// workgroup: heights of many tiny supernodes leave most of a workgroup idle and
+// pay a workgroup barrier per phase, while a wavefront needs none. Every row's
+// sums are formed exactly as above (G lanes per row, entries k = lane + u G in
+// ascending u, then the lane-group tree; backward, the four row classes a mod 4
+// in ascending order, combined as ((0 + 1) + (2 + 3))), so both forms give the
+// same bits and the tuner may pick either per height.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_fwd_wave(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                      const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                      const int* __restrict__ sz, const int* __restrict__ nr,
+                                                      const long long* __restrict__ doff, const float* __restrict__ diag,
+                                                      const int* __restrict__ inc_ptr, const long long* __restrict__ cpos,
+                                                      const int* __restrict__ orig_of, const float* __restrict__ b,
+                                                      const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                      const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
+                                                      const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    __shared__ __align__(16) float ts_all[4][TriPlanHost::kSliceRows][M];
+    __shared__ __align__(16) float ys_all[4][TriPlanHost::kSliceRows][M];
+    __shared__ int base_all[4][M];
+    const int w = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
+    const int run = blockIdx.x * 4 + w;
+    if (run >= nruns) return;
+    float(*ts)[M] = ts_all[w];
+    float(*ys)[M] = ys_all[w];
+    int* base_s = base_all[w];
+    const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
+    const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
+    bool any = false;
+    for (int j = 0; j < cnt; ++j) any = any || !sn_stopped(sn_prob, active, sn_list[q0 + j]);
+    if (!any) return;
+    if (lane64 < cnt) base_s[lane64] = s0[sn_list[q0 + lane64]];
+    wave_sync();
+    for (int idx = lane64; idx < M * n; idx += 64) {
+        const int j = idx / n, k = idx - j * n;
+        float v = 0.f;
+        if (j < cnt) {
+            const int i = base_s[j] + k;
+            float sum = 0.f;
+#pragma unroll 8
+            for (int p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) sum += c[cpos[p]];
+            v = b[orig_of[i]] - sum;
+        }
+        ts[k][j] = v;
+    }
+    wave_sync();
+    const float* D = diag + doff[g0];
+    const int lane = lane64 & (G - 1);
+    constexpr int kU = TriPlanHost::kSliceRows / G;
+    for (int r = lane64 / G; r < n; r += 64 / G) {
+        const float* Dr = D + (long long)r * n;
+        float acc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] = 0.f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = lane + u * G;
+            if (k <= r) {
+                const float d = Dr[k];
+                float tk[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) acc[j] += d * tk[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < cnt) {
+                const float v = group_sum<G>(acc[j]);
+                if (lane == 0) { ys[r][j] = v; y[base_s[j] + r] = v; }
+            } else if (lane == 0) {
+                ys[r][j] = 0.f;
+            }
+    }
+    wave_sync();
+    const float* L = l21 + loff[g0];
+    for (int a = lane64 / G; a < m; a += 64 / G) {
+        const float* La = L + (long long)a * n;
+        float acc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] = 0.f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = lane + u * G;
+            if (k < n) {
+                const float l = La[k];
+                float yk[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) yk[j] = ys[k][j];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) acc[j] += l * yk[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < cnt) {
+                const float v = group_sum<G>(acc[j]);
+                if (lane == 0) c[cofs[sn_list[q0 + j]] + a] = v;
+            }
+    }
+}
+
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_bwd_wave(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                      const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                      const int* __restrict__ sz, const int* __restrict__ nr,
+                                                      const int* __restrict__ roff, const int* __restrict__ srows,
+                                                      const long long* __restrict__ loff, const float* __restrict__ l21,
+                                                      const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                      const int* __restrict__ orig_of, const float* __restrict__ y,
+                                                      float* __restrict__ z, float* __restrict__ x,
+                                                      const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    constexpr int kChunk = 128;
+    __shared__ __align__(16) float zr_all[4][kChunk][M];
+    __shared__ __align__(16) float ts_all[4][TriPlanHost::kSliceRows][M];
+    __shared__ int base_all[4][M], roff_all[4][M];
+    const int w = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
+    const int run = blockIdx.x * 4 + w;
+    if (run >= nruns) return;
+    float(*zr)[M] = zr_all[w];
+    float(*ts)[M] = ts_all[w];
+    int* base_s = base_all[w];
+    int* roff_s = roff_all[w];
+    const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
+    const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
+    bool any = false;
+    for (int j = 0; j < cnt; ++j) any = any || !sn_stopped(sn_prob, active, sn_list[q0 + j]);
+    if (!any) return;
+    if (lane64 < cnt) { base_s[lane64] = s0[sn_list[q0 + lane64]]; roff_s[lane64] = roff[sn_list[q0 + lane64]]; }
+    const float* L = l21 + loff[g0];
+    const int j0 = min(lane64, n - 1), j1 = min(lane64 + 64, n - 1);
+    // acc0[c][j] / acc1[c][j]: the sums over the rows a = c mod 4 of columns j0 / j1
+    float acc0[4][M], acc1[4][M];
+#pragma unroll
+    for (int cl = 0; cl < 4; ++cl)
+#pragma unroll
+        for (int j = 0; j < M; ++j) { acc0[cl][j] = 0.f; acc1[cl][j] = 0.f; }
+    wave_sync();
+    for (int a0 = 0; a0 < m; a0 += kChunk) {
+        const int cm = min(kChunk, m - a0);
+        for (int idx = lane64; idx < M * cm; idx += 64) {
+            const int j = idx / cm, q = idx - j * cm;
+            zr[q][j] = j < cnt ? z[srows[roff_s[j] + a0 + q]] : 0.f;
+        }
+        wave_sync();
+        for (int q = 0; q < cm; q += 4) {
+#pragma unroll
+            for (int cl = 0; cl < 4; ++cl) {
+                if (q + cl >= cm) break;
+                const float* La = L + (long long)(a0 + q + cl) * n;
+                const float l0 = La[j0], l1 = La[j1];
+                float zq[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) zq[j] = zr[q + cl][j];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) {
+                        acc0[cl][j] += l0 * zq[j];
+                        acc1[cl][j] += l1 * zq[j];
+                    }
+            }
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        if (lane64 < n) ts[lane64][j] = j < cnt ? y[base_s[j] + lane64] - ((acc0[0][j] + acc0[1][j]) + (acc0[2][j] + acc0[3][j])) : 0.f;
+        if (lane64 + 64 < n)
+            ts[lane64 + 64][j] = j < cnt ? y[base_s[j] + lane64 + 64] - ((acc1[0][j] + acc1[1][j]) + (acc1[2][j] + acc1[3][j])) : 0.f;
+    }
+    wave_sync();
+    const float* DT = diagT + doff[g0];
+    const int gl = lane64 & (G - 1);
+    constexpr int kU = TriPlanHost::kSliceRows / G;
+    for (int r = lane64 / G; r < n; r += 64 / G) {
+        const float* Dr = DT + (long long)r * n;
+        float acc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) acc[j] = 0.f;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int k = r + gl + u * G;
+            if (k < n) {
+                const float d = Dr[k];
+                float tk[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < cnt) acc[j] += d * tk[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < cnt) {
+                const float v = group_sum<G>(acc[j]);
+                if (gl == 0) { z[base_s[j] + r] = v; x[orig_of[base_s[j] + r]] = v; }
+            }
+    }
+}
+
 // y = A x for CSR A (one thread per row, column order)
 __global__ void k_csr_mv(int rows, const int* __restrict__ p, const int* __restrict__ ix, const float* __restrict__ v,
                          const float* __restrict__ x, float* __restrict__ y) {
@@ -683,6 +893,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
     sn_begin = P.sn_begin; sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
     ci_entries = P.ci_entries; h_sn_list = P.sn_list; h_sz = P.sz; h_nr = P.nr;
     run_begin = P.run_begin; run_max = P.run_max;
+    wave.assign(P.H + 1, 0);
     run_q.upload(P.run_q); run_n.upload(P.run_n);
     max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
     orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
@@ -756,7 +967,18 @@ void TriPlan::fwd_height(int h, const float* b, hipStream_t s) {
     {
         if (fused[h]) {
             const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
-            if (nr_) {
+            if (nr_ && wave[h]) {
+                if (run_max[h] > 1)
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                                     (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p,
+                                     (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b,
+                                     (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
+                else
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                                     (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p,
+                                     (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b,
+                                     (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
+            } else if (nr_) {
                 if (run_max[h] > 1)
                     HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
                                      (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
@@ -796,7 +1018,18 @@ void TriPlan::bwd_height(int h, float* x, hipStream_t s) {
     {
         if (fused[h]) {
             const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
-            if (nr_) {
+            if (nr_ && wave[h]) {
+                if (run_max[h] > 1)
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                                     (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p,
+                                     (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
+                                     (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
+                else
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                                     (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p,
+                                     (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
+                                     (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
+            } else if (nr_) {
                 if (run_max[h] > 1)
                     HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
                                      (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
@@ -925,7 +1158,17 @@ int TriPlan::tune(hipStream_t s) {
     };
     int changed = 0;
     for (int h = 0; h <= H; ++h) {
-        if (fused[h] || sl_begin[h + 1] == sl_begin[h]) continue;
+        if (fused[h]) {
+            // a workgroup or a wavefront per run: the same bits, whichever is faster
+            if (run_begin[h + 1] == run_begin[h]) continue;
+            const float t_wg = time_height(h);
+            wave[h] = 1;
+            const float t_wave = time_height(h);
+            if (!(t_wave < 0.97f * t_wg)) wave[h] = 0;
+            changed += wave[h];
+            continue;
+        }
+        if (sl_begin[h + 1] == sl_begin[h]) continue;
         const int sl0 = sl_rows[h], ci0 = ci_entries[h];
         float best = time_height(h);
         for (int cand : {8, 16, 32, 64, 128}) {

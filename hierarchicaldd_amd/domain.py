@@ -129,13 +129,17 @@ def _padded(n):
     return (-(-n // BX) * BX, -(-n // BY) * BY, -(-n // BZ) * BZ)
 
 
-def box_flags(n: int, spheres, drop_enclosed: bool = False) -> np.ndarray:
+def box_flags(n: int, spheres, drop_enclosed: bool = False, obstacle_boxes=(), dirichlet_ring: bool = True) -> np.ndarray:
     """Dense flags (padded to page multiples) of an n^3 box with a Dirichlet ring
     and Neumann sphere obstacles (``spheres``: list of (cx, cy, cz, r)).
 
     ``drop_enclosed``: Active pockets sealed off from every Dirichlet cell by
     obstacles (pure-Neumann, singular subproblems) are turned into obstacle
-    cells, so any right-hand side is solvable."""
+    cells, so any right-hand side is solvable.
+    ``obstacle_boxes``: further Neumann obstacles, each ((x0,y0,z0), (x1,y1,z1))
+    with inclusive cell bounds.
+    ``dirichlet_ring=False``: the ring is exterior (flag 0) instead, so the
+    whole domain is pure Neumann and the operator singular."""
     X, Y, Z = _padded(n)
     i = np.arange(X)[:, None, None]
     j = np.arange(Y)[None, :, None]
@@ -145,8 +149,11 @@ def box_flags(n: int, spheres, drop_enclosed: bool = False) -> np.ndarray:
     obstacle = np.zeros((X, Y, Z), dtype=bool)
     for (cx, cy, cz, r) in spheres:
         _mark_sphere(obstacle, cx, cy, cz, r)
+    for (lo, hi) in obstacle_boxes:
+        obstacle[lo[0]:hi[0] + 1, lo[1]:hi[1] + 1, lo[2]:hi[2] + 1] = True
     flags = np.zeros((X, Y, Z), dtype=np.uint32)
-    flags[inside & ~interior] = DIRICHLET
+    if dirichlet_ring:
+        flags[inside & ~interior] = DIRICHLET
     active = interior & ~obstacle
     if drop_enclosed:
         active = _connected_to_ring(active, n)
@@ -299,6 +306,30 @@ def small_box(n: int = 32, sub: int = 16, vlevels: int = 3, seed: int = 0, sigma
     rhs = rng.standard_normal((n, n, n)).astype(np.float32)
     d = sphere_obstacle_box(n, [(c + 1, c - 2, c, max(2, n // 6))], rhs=rhs, meta={"workload": f"box{n}_sub{sub}"})
     return d, DDConfig(sub_size=sub, vlevels=vlevels, sigma_levels=sigma_levels)
+
+
+def neumann_box(n: int = 32, sub: int = 16, vlevels: int = 3, seed: int = 0):
+    """Pure-Neumann problem: every interior cell of an n^3 box Active, no Dirichlet
+    cell anywhere (the operator is singular, A 1 = 0); an N(0,1) right-hand side
+    with its mean removed, so it lies in the range of A."""
+    rng = np.random.default_rng(seed)
+    flags = box_flags(n, [], dirichlet_ring=False)
+    rhs = rng.standard_normal((n, n, n))
+    act = (flags[:n, :n, :n] & ACTIVE) != 0
+    rhs[act] -= rhs[act].mean()
+    d = domain_from_flags(n, flags, rhs=rhs.astype(np.float32), meta={"workload": f"neumann{n}_sub{sub}"})
+    return d, DDConfig(sub_size=sub, vlevels=vlevels)
+
+
+def empty_subdomain_box(n: int = 48, sub: int = 16, vlevels: int = 3, seed: int = 0):
+    """A box whose subdomain (1,1,1) (cells sub..2 sub) holds no Active cell: a cube
+    obstacle fills its interior, so the divider gives it no interior dofs. N(0,1)
+    right-hand side on the Active cells."""
+    rng = np.random.default_rng(seed)
+    flags = box_flags(n, [], obstacle_boxes=[((sub + 1,) * 3, (2 * sub - 1,) * 3)])
+    rhs = rng.standard_normal((n, n, n)).astype(np.float32)
+    d = domain_from_flags(n, flags, rhs=rhs, meta={"workload": f"emptysub{n}_sub{sub}"})
+    return d, DDConfig(sub_size=sub, vlevels=vlevels)
 
 
 def batch_problems(count: int = 64, seed: int = 0, sub: int = 32, vlevels: int = 4):

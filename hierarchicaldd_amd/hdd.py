@@ -35,7 +35,7 @@ class PcgParams(C.Structure):
 
 
 class PcgReport(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("final_norm", C.c_double)]
+    _fields_ = [("iterations", C.c_int), ("seconds", C.c_double), ("final_norm", C.c_double), ("converged", C.c_int)]
 
 
 class ServerParams(C.Structure):
@@ -44,7 +44,8 @@ class ServerParams(C.Structure):
 
 class RequestReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("final_norm", C.c_double), ("latency_s", C.c_double), ("setup_s", C.c_double),
-                ("queue_s", C.c_double), ("solve_s", C.c_double), ("batch_size", C.c_int)]
+                ("queue_s", C.c_double), ("solve_s", C.c_double), ("batch_size", C.c_int), ("concurrent", C.c_int),
+                ("cancelled", C.c_int)]
 
 
 EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error", "hdd_num_pages", "hdd_page_offsets", "hdd_set_channel",
@@ -54,7 +55,7 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_bottom_spectrum", "hdd_profile_copy", "hdd_sigma_plan_stats",
            "hdd_profile_solve", "hdd_create_batch", "hdd_batch_count", "hdd_batch_num_pages", "hdd_batch_page_offsets",
            "hdd_batch_pcg_solve", "hdd_batch_get_x", "hdd_create_batch_from", "hdd_server_create", "hdd_server_submit",
-           "hdd_server_wait", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph", "hdd_set_vcycle_storage",
+           "hdd_server_wait", "hdd_server_cancel", "hdd_server_destroy", "hdd_server_last_error", "hdd_server_budget", "hdd_set_graph", "hdd_set_vcycle_storage",
            "hdd_set_sigma_refinement", "hdd_sigma_factor_solve", "hdd_set_stream", "hdd_synchronize", "hdd_multiply",
            "hdd_inner_product", "hdd_convergence_norm", "hdd_project", "hdd_vector_copy", "hdd_vector_copy2",
            "hdd_vector_plus_equal", "hdd_vector_minus_equal", "hdd_vector_scale", "hdd_apply_preconditioner_channels",
@@ -107,6 +108,7 @@ def lib():
         L.hdd_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(C.c_void_p)]
         L.hdd_server_submit.argtypes = [P, C.POINTER(ProblemDesc), C.POINTER(PcgParams), C.POINTER(C.c_int64)]
         L.hdd_server_wait.argtypes = [P, C.c_int64, P, C.POINTER(RequestReport)]
+        L.hdd_server_cancel.argtypes = [P, C.c_int64]
         L.hdd_server_destroy.argtypes = [P]
         L.hdd_server_last_error.argtypes = [P]
         L.hdd_server_last_error.restype = C.c_char_p
@@ -302,6 +304,16 @@ class DDSolver:
         n = abs(rep.iterations) + 1
         return rep.iterations, hist[:n], rep.seconds
 
+    def solve_status(self, tolerance: float, max_iterations: int = 5000):
+        """PCG from x=0; returns (status code, report, residual history) without raising on
+        HDD_ERR_NOT_CONVERGED (status 4)."""
+        p = PcgParams(tolerance, max_iterations)
+        rep = PcgReport()
+        hist = np.zeros(max_iterations + 1, dtype=np.float64)
+        st = lib().hdd_pcg_solve(self.h, C.byref(p), C.byref(rep), _ptr(hist), max_iterations + 1)
+        self._check(st, ok=(0, 4))
+        return st, rep, hist[:abs(rep.iterations) + 1]
+
     def profile_sweep(self, reps: int = 50):
         """Mean duration (ms) of one level-0 smoothing sweep and its algorithmic bytes."""
         ms, by = C.c_double(), C.c_double()
@@ -330,11 +342,12 @@ class DDSolver:
 
     def plan_stats(self, level: int, which: str):
         """Per tree height of a triangular-solve plan ("Aii" batch or "Arr"): supernodes, rows,
-        packed diagonal entries, L21 entries, largest supernode."""
-        out = np.zeros(5 * 64, dtype=np.float64)
+        packed diagonal entries, L21 entries, largest supernode, most L21 rows of a supernode,
+        most incidences of a row, all incidences."""
+        out = np.zeros(8 * 64, dtype=np.float64)
         h = C.c_int()
         self._check(lib().hdd_sigma_plan_stats(self.h, level, {"Aii": 0, "Arr": 1}[which], _ptr(out), out.size, C.byref(h)))
-        return out[:5 * min(h.value, 64)].reshape(-1, 5)
+        return out[:8 * min(h.value, 64)].reshape(-1, 8)
 
     def bottom_spectrum(self, lambda_cut: float, cap: int = 1024):
         """Eigenvalues >= lambda_cut of the bottom sigma smoother's iteration matrix (host, double)."""
@@ -404,9 +417,12 @@ class DDBatch(DDSolver):
 
 
 class Server:
-    """Serving loop (hdd_server_*): submit problems as they arrive, wait for each result."""
+    """Serving loop (hdd_server_*): submit problems as they arrive, wait for each result.
+    Requests ready together form a unit (one batched context); up to `groups` units iterate
+    on the device at once, and a unit admitted while others are mid-solve joins them at
+    their next iteration (continuous batching)."""
 
-    def __init__(self, setup_threads: int = 2, groups: int = 2, max_batch: int = 16, hbm_budget: float = 0.0):
+    def __init__(self, setup_threads: int = 2, groups: int = 4, max_batch: int = 16, hbm_budget: float = 0.0):
         L = lib()
         prm = ServerParams(setup_threads, groups, max_batch, float(hbm_budget))
         h = C.c_void_p()
@@ -437,10 +453,15 @@ class Server:
         x = np.zeros((self.n_pages.pop(rid), 256), dtype=np.float32)
         rep = RequestReport()
         st = lib().hdd_server_wait(self.h, rid, _ptr(x), C.byref(rep))
-        if st not in (0, 4):
+        if st not in (0, 4, 5):
             raise HddError(f"request {rid} failed ({st}): {lib().hdd_server_last_error(self.h).decode()}")
         info = {k: getattr(rep, k) for k, _ in RequestReport._fields_}
         return st, x, info
+
+    def cancel(self, rid: int) -> bool:
+        """Asks request rid to stop (dropped if queued, stopped at its next residual test if
+        running); False if it is unknown or already done. Its wait() then returns status 5."""
+        return lib().hdd_server_cancel(self.h, int(rid)) == 0
 
     def close(self):
         if getattr(self, "h", None):

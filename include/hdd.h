@@ -27,7 +27,7 @@ extern "C" {
 #endif
 
 typedef int hdd_status;
-enum { HDD_OK = 0, HDD_ERR_ARG = 1, HDD_ERR_SETUP = 2, HDD_ERR_DEVICE = 3, HDD_ERR_NOT_CONVERGED = 4 };
+enum { HDD_OK = 0, HDD_ERR_ARG = 1, HDD_ERR_SETUP = 2, HDD_ERR_DEVICE = 3, HDD_ERR_NOT_CONVERGED = 4, HDD_ERR_CANCELLED = 5 };
 
 typedef struct hdd_ctx hdd_ctx;
 
@@ -58,6 +58,7 @@ typedef struct hdd_pcg_report {
     int iterations;        /* negative if not converged within max_iterations */
     double seconds;        /* wall time of the solve on the context stream */
     double final_norm;     /* max|r| at exit */
+    int converged;         /* 1 if max|r| <= tolerance at exit (the status is HDD_OK exactly then) */
 } hdd_pcg_report;
 
 /* Replaces SPGrid_Domain_Decomposition_Solver::Read_SPGrid + Initialize_Preconditioner
@@ -330,11 +331,13 @@ hdd_status hdd_profile_solve(hdd_ctx* ctx, int level, int which, int reps, doubl
 
 /* Shape of the triangular-solve plan of sigma level `level`: which = 0 for
  * the batch of all subdomains' (Aii + eps I), 1 for (Arr + eps I). Factors on
- * the host (works on host-only contexts) and writes 5 values per tree height
- * h into out[5h .. 5h+4]: supernodes, rows, packed diagonal-inverse entries,
- * sub-diagonal (L21) entries, largest supernode; *heights = tree heights (only
- * the first cap/5 are written). Used to state the algorithmic bytes of the
- * solve kernels. No reference counterpart. */
+ * the host (works on host-only contexts) and writes 8 values per tree height
+ * h into out[8h .. 8h+7]: supernodes, rows, packed diagonal-inverse entries,
+ * sub-diagonal (L21) entries, largest supernode, most L21 rows of one
+ * supernode, most incidences (contributions of descendants) of one row, all
+ * incidences of the height's rows; *heights = tree heights (only the first
+ * cap/8 are written). Used to state the algorithmic bytes and the largest work
+ * items of the solve kernels. No reference counterpart. */
 hdd_status hdd_sigma_plan_stats(hdd_ctx* ctx, int level, int which, double* out, int cap, int* heights);
 
 /* x = (A + 1e-5 I)^{-1} b in double precision with the host factor of sigma
@@ -354,30 +357,36 @@ hdd_status hdd_sigma_factor_solve(hdd_ctx* ctx, int level, int which, int subdom
 hdd_status hdd_bottom_spectrum(hdd_ctx* ctx, double lambda_cut, double* lambdas, int cap, int* k, double* lambda_max);
 
 /* ---- Serving (no reference counterpart: the reference solves one problem per
- * process). Requests of different sizes are submitted as they arrive; worker
- * threads set each one up on the host (as hdd_create) and admit it to the GPU
- * in groups, each group one batched context (as hdd_create_batch) on its own
- * stream, several groups side by side, within an HBM budget. A request's
- * device state stays resident until it has converged; its result is returned
- * as soon as it stops. Requests in one group need the same vlevels,
- * sigma_levels and smoothing counts (others wait for a later group). */
+ * process). Requests of different sizes are submitted as they arrive and solved
+ * with continuous batching: worker threads set each one up on the host (as
+ * hdd_create); a builder admits the ready requests, within an HBM budget, as a
+ * unit (one batched context, as hdd_create_batch, on its own stream); a driver
+ * thread advances every resident unit one PCG iteration at a time, so units
+ * admitted while others are mid-solve join at the next iteration and their
+ * kernels overlap on the device. A request's device state stays resident until
+ * it stops; its result is returned as soon as it does, and a unit's memory and
+ * slot are freed when all its requests have stopped. Requests in one unit need
+ * the same vlevels, sigma_levels and smoothing counts (others wait for a later
+ * unit). */
 typedef struct hdd_server hdd_server;
 
 typedef struct hdd_server_params {
     int setup_threads;        /* host setup workers */
-    int groups;               /* groups solved side by side on the device */
-    int max_batch;            /* problems per group */
+    int groups;               /* units resident and iterating on the device at once */
+    int max_batch;            /* problems per unit */
     double hbm_budget_bytes;  /* device bytes for resident problems; 0 = 90% of the free device memory */
 } hdd_server_params;
 
 typedef struct hdd_request_report {
-    int iterations;           /* negative if not converged within max_iterations */
+    int iterations;           /* negative if it stopped before converging */
     double final_norm;        /* max|r| at exit */
     double latency_s;         /* submit -> result */
     double setup_s;           /* submit -> host setup done (includes waiting for a worker) */
-    double queue_s;           /* host setup done -> admitted to a device group */
-    double solve_s;           /* admitted -> result */
-    int batch_size;           /* problems in its group */
+    double queue_s;           /* host setup done -> its unit's first iteration (includes the device build) */
+    double solve_s;           /* its unit's first iteration -> result */
+    int batch_size;           /* problems in its unit */
+    int concurrent;           /* units iterating on the device when its unit started, its own included */
+    int cancelled;            /* 1 if it stopped because of hdd_server_cancel */
 } hdd_request_report;
 
 hdd_status hdd_server_create(const hdd_server_params* params, hdd_server** out);
@@ -388,8 +397,16 @@ hdd_status hdd_server_submit(hdd_server* srv, const hdd_problem_desc* desc, cons
 /* Blocks until the request is done; x (may be NULL) receives n_pages*256
  * floats in the page order of its desc. Returns HDD_ERR_SETUP / HDD_ERR_DEVICE
  * with hdd_server_last_error() set if the request failed, HDD_ERR_NOT_CONVERGED
- * if it stopped at its iteration limit. The request's record is released. */
+ * if it stopped at its iteration limit, HDD_ERR_CANCELLED if it was cancelled
+ * (x then holds the iterate it stopped at, or zeros if it never ran). The
+ * request's record is released. */
 hdd_status hdd_server_wait(hdd_server* srv, int64_t request_id, float* x, hdd_request_report* report);
+
+/* Asks a request to stop: one still queued is dropped before it reaches the
+ * device; one that is running stops at its unit's next residual test (the
+ * other requests of its unit continue). HDD_ERR_ARG if the request is unknown
+ * or already done. */
+hdd_status hdd_server_cancel(hdd_server* srv, int64_t request_id);
 
 /* Finishes every submitted request, then stops the workers and frees the server. */
 hdd_status hdd_server_destroy(hdd_server* srv);

@@ -1,0 +1,80 @@
+"""Failure paths of the GPU solve, each against the oracle (the CPU restatement
+of the reference dd_pcg) on the same input:
+
+* non-convergence: max_iterations 0 and 3 give HDD_ERR_NOT_CONVERGED with
+  converged = 0 and |iterations| = max_iterations, and the residual history of
+  the iterations that ran equals the oracle's (the reference returns false
+  after max_iterations, DDS/CONJUGATE_GRADIENT_DUMPING.cpp:188-191,207-211);
+* a pure-Neumann domain (no Dirichlet cell anywhere, so A 1 = 0) with a
+  right-hand side in the range of A: both converge, iteration counts within
+  +-1, histories within the single-precision bar of tests/test_gpu_pcg.py;
+* a subdomain with no Active cell (the reference skips it,
+  IFS/SIGMA_SMOOTHER.cpp:308): same bars.
+
+A domain with no interface at all (edge = sub_size) runs in
+tests/test_gpu_batch.py.
+"""
+import numpy as np
+import pytest
+
+from hierarchicaldd_amd import domain
+
+pytestmark = pytest.mark.gpu
+
+HISTORY_BAR_SINGLE = 5e-4
+HDD_OK, HDD_ERR_NOT_CONVERGED = 0, 4
+
+
+def _history_rel(h_g, h_o):
+    n = min(len(h_o), len(h_g))
+    return float((np.abs(h_g[:n] - h_o[:n]) / np.maximum(h_o[:n], 1e-30)).max())
+
+
+@pytest.mark.parametrize("max_it", [0, 3])
+def test_not_converged(max_it, oracle_lib):
+    from hierarchicaldd_amd import hdd
+    dom, cfg = domain.small_box(32, 16, 3, seed=1)
+    print(f"[not converged] box32_sub16 max_it={max_it}", flush=True)
+    tol = 1e-6 * float(np.abs(dom.b).max())
+    o = oracle_lib.Oracle(dom, cfg)
+    it_o, h_o, _ = o.solve(tol, max_it)
+    g = hdd.DDSolver(dom, cfg)
+    try:
+        st, rep, h_g = g.solve_status(tol, max_it)
+    finally:
+        g.close()
+    assert it_o == -max_it and len(h_o) == max_it + 1
+    assert st == HDD_ERR_NOT_CONVERGED
+    assert rep.converged == 0
+    assert rep.iterations == -max_it
+    assert len(h_g) == max_it + 1 and h_g[-1] > tol
+    assert rep.final_norm == pytest.approx(h_g[-1])
+    assert _history_rel(h_g, h_o) < HISTORY_BAR_SINGLE
+
+
+CASES = {
+    "neumann32_sub16": lambda: domain.neumann_box(32, 16, 3, seed=0),
+    "emptysub48_sub16": lambda: domain.empty_subdomain_box(48, 16, 3, seed=0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_degenerate_domain(name, oracle_lib):
+    from hierarchicaldd_amd import hdd
+    dom, cfg = CASES[name]()
+    print(f"[degenerate] {name} pages={dom.n_pages}", flush=True)
+    tol = 1e-6 * float(np.abs(dom.b).max())
+    o = oracle_lib.Oracle(dom, cfg)
+    it_o, h_o, _ = o.solve(tol, 200)
+    g = hdd.DDSolver(dom, cfg)
+    try:
+        st, rep, h_g = g.solve_status(tol, 200)
+        x = g.get(hdd.CH_X)
+    finally:
+        g.close()
+    rel = _history_rel(h_g, h_o)
+    print(f"[{name}] oracle iters {it_o} gpu iters {rep.iterations} history max rel diff {rel:.3e}")
+    assert st == HDD_OK and rep.converged == 1
+    assert it_o > 0 and abs(rep.iterations - it_o) <= 1
+    assert np.isfinite(x).all()
+    assert rel < HISTORY_BAR_SINGLE

@@ -1,11 +1,15 @@
 """Serving loop: requests of different sizes submitted one after another, set up
-on host workers, admitted to the GPU in groups that run side by side.
+on host workers, admitted to the GPU as units that iterate side by side
+(continuous batching: a unit admitted while others are mid-solve joins them
+at their next iteration).
 
 Bar: each request's result in the mix is the result it gets solved alone
 (tolerances as for batched solves: iteration count within +-1, solution
-within 1e-4 relative in max-norm), and every result meets its own tolerance.
-Also: a request larger than the HBM budget is rejected with an error, and
-requests with different V-cycle schedules are solved in separate groups.
+within 1e-4 relative in max-norm; a unit of one request gives the same bits
+as the solve alone), and every result meets its own tolerance. Also: a
+request that joins while another is mid-solve, cancellation of a running and
+of a queued request, a request larger than the HBM budget rejected with an
+error, and requests with different V-cycle schedules solved in separate units.
 """
 import time
 
@@ -92,3 +96,56 @@ def test_server_mixed_schedules():
     srv.close()
     assert sa == 0 and sb == 0
     assert ra["batch_size"] == 1 and rb["batch_size"] == 1
+
+
+def test_server_continuous_batching_and_cancel():
+    """Two requests that never converge (tolerance 0) hold both device slots; probe
+    requests submitted meanwhile queue, and one cancelled while queued never runs.
+    Then one long request is cancelled mid-flight: the probes join the other one
+    mid-solve (concurrent units >= 2) and each gives the bits of its solve alone."""
+    from hierarchicaldd_amd import hdd
+    from tests.gpu_common import same_bits
+    long_dom, long_cfg = domain.small_box(48, 16, 3, seed=41)
+    probes = [domain.small_box(32, 16, 3, seed=42 + k) for k in range(3)]
+    alone = []
+    for dom, cfg in probes:
+        g = hdd.DDSolver(dom, cfg)
+        it, _, _ = g.solve(_tol(dom), 300)
+        alone.append((it, g.get(hdd.CH_X)))
+        g.close()
+    srv = hdd.Server(setup_threads=2, groups=2, max_batch=1)
+    try:
+        long_a = srv.submit(long_dom, long_cfg, 0.0, 1000000)
+        long_b = srv.submit(long_dom, long_cfg, 0.0, 1000000)
+        time.sleep(2.0)  # both long requests set up and iterating
+        queued = srv.submit(*probes[0], _tol(probes[0][0]), 300)
+        assert srv.cancel(queued)
+        st_q, x_q, rep_q = srv.wait(queued)
+        print(f"[queued request] status {st_q} iterations {rep_q['iterations']} concurrent {rep_q['concurrent']}", flush=True)
+        assert st_q == 5 and rep_q["cancelled"] == 1
+        assert rep_q["solve_s"] == 0.0 and not x_q.any()
+        # free one slot mid-flight: the probes are admitted one unit each beside long_b
+        assert srv.cancel(long_a)
+        st_a, _, rep_a = srv.wait(long_a)
+        print(f"[long request a] status {st_a} iterations {rep_a['iterations']} solve {rep_a['solve_s']:.3f}s", flush=True)
+        assert st_a == 5 and rep_a["cancelled"] == 1 and rep_a["iterations"] < 0
+        ids = [srv.submit(dom, cfg, _tol(dom), 300) for dom, cfg in probes]
+        out = [srv.wait(i) for i in ids]
+        assert srv.cancel(long_b)
+        st_b, _, rep_b = srv.wait(long_b)
+        assert st_b == 5 and rep_b["iterations"] < 0
+    finally:
+        # requests that never converge must not outlive the test (close() finishes every request)
+        srv.cancel(long_a)
+        srv.cancel(long_b)
+        srv.close()
+    joined = 0
+    for k, ((dom, _), (st, x, rep), (it_a, x_a)) in enumerate(zip(probes, out, alone)):
+        bits = same_bits(x, x_a)
+        print(f"[probe {k}] alone {it_a} served {rep['iterations']}, concurrent units {rep['concurrent']}, "
+              f"bit-identical {bits}, x rel diff {rel_err(x, x_a):.2e}", flush=True)
+        assert st == 0 and rep["iterations"] == it_a
+        assert bits
+        joined += rep["concurrent"] >= 2
+    # long_b was mid-solve (it never converges) while every probe ran
+    assert joined == len(probes)
