@@ -309,3 +309,35 @@ def test_interface_prolongation_is_the_trilinear_node_rule(case, oracle_lib):
         assert np.all(P[full].sum(axis=1) == 1.0)
         checked += r
     assert checked > 0
+
+
+# ---- Galerkin consistency of the adaptive interface operators: with no Dirichlet
+# cell anywhere the fine operator annihilates constants, and so must every sigma
+# level's combined system [[Aii_s, Air_s], [Air_s^T, Arr]] built by Galerkin
+# coarsening (GALERKIN_PROCESS.h:27-68, CELL_PROLONGATION_BAKER.h:22-51) on the
+# adaptive stride lattice (STRIDE_GENERATOR.h:23-54) with T-junction nodes embedded
+# into their parents (INTERFACE_MATRIX_GENERATOR.h:81-100): its prolongations
+# reproduce constants only if every child weight set and every embedding sums to 1.
+
+@pytest.mark.parametrize("n,sub,vlevels", [(32, 16, 3), (64, 32, 4)])
+def test_pure_neumann_operators_annihilate_constants(n, sub, vlevels, oracle_lib):
+    dom, cfg = domain.neumann_box(n, sub, vlevels, seed=0)
+    o = oracle_lib.Oracle(dom, cfg)
+    for l in range(o.sigma_levels):
+        Arr = _csr(o.matrix("Arr", l))
+        ones_if = np.ones(Arr.shape[0])
+        if_sum = Arr @ ones_if
+        scale = np.abs(Arr.diagonal()).max()
+        for s in range(o.n_subdomains):
+            Aii = _csr(o.matrix("Aii", l, s))
+            if Aii.shape[0] == 0:
+                continue
+            Air = _csr(o.matrix("Air", l, s))
+            assert Air.shape == (Aii.shape[0], Arr.shape[0])
+            rows = Aii @ np.ones(Aii.shape[0]) + Air @ ones_if
+            assert np.abs(rows).max() <= 1e-6 * scale, (l, s, np.abs(rows).max())
+            if_sum = if_sum + Air.T @ np.ones(Aii.shape[0])
+        assert np.abs(if_sum).max() <= 1e-6 * scale, (l, np.abs(if_sum).max())
+        # and a non-constant field is not annihilated (the check has teeth)
+        ramp = np.asarray(o.interface_coords(l), dtype=np.float64)[:, 0]
+        assert np.abs(Arr @ ramp).max() > 1e-3 * scale
