@@ -3,7 +3,10 @@ hand-derived values of the reference's stride generator, the transfer
 stencils' weights and constants against a dense numpy statement of the
 full-weighting restriction and trilinear prolongation, the damped-Jacobi
 constant and face count on the oracle's own masks, the boundary-band flags
-against a brute-force distance, and the sparse Cholesky solves against scipy.
+against a brute-force distance, the sparse Cholesky solves against scipy, the
+interface prolongation against the trilinear node rule, the Galerkin-coarsened
+interface operators against the constant null vector of a pure-Neumann domain,
+and the coarse V-cycle flags against the downsampling rule.
 
 Reference lines each check follows:
   STRIDE_GENERATOR::Generate_Adaptive_Stride   IFS/STRIDE_GENERATOR.h:23-54
@@ -12,6 +15,10 @@ Reference lines each check follows:
   Residual                                     LIB/SPGrid_Residual_Legacy.h:17-62
   Correction (omega = 2/3, 1/face_count)       LIB/SPGrid_Correction_Legacy.h:35,60-77
   Flaging_Boundary (3 in-place sweeps, <= 3)   DDS/SPGrid_Flag_Boundary_Helper.h:81-107
+  Construct_Prolongation_Matrix (Prr)          IFS/PROLONGATION_MATRIX_HELPER.cpp:13-40
+  Galerkin_Coarsen, Bake, Embedding_Helper     IFS/GALERKIN_PROCESS.h:27-68, IFS/CELL_PROLONGATION_BAKER.h:22-51,
+                                               IFS/INTERFACE_MATRIX_GENERATOR.h:81-100
+  Topology_Downsample                          DDS/SPGrid_V_Cycle_Topology.h:35-67
 (IFS/, LIB/, DDS/ = demos/Domain_Decomposition_CPU_Only/{Interface_Solver,
 SPGrid_Library, SPGrid_Domain_Decomposition_Solver}/)
 """
@@ -22,7 +29,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
 from hierarchicaldd_amd import domain
-from tests.reference_ops import FACES, from_dense, to_dense
+from tests.reference_ops import FACES, cell_coords, from_dense, to_dense
 
 PAD = 4
 
@@ -341,3 +348,39 @@ def test_pure_neumann_operators_annihilate_constants(n, sub, vlevels, oracle_lib
         # and a non-constant field is not annihilated (the check has teeth)
         ramp = np.asarray(o.interface_coords(l), dtype=np.float64)[:, 0]
         assert np.abs(Arr @ ramp).max() > 1e-3 * scale
+
+
+# ---- Topology_Downsample (DDS/SPGrid_V_Cycle_Topology.h:35-67): every fine cell f maps
+# to the coarse cell c = (f + 1) / 2 per axis; c is Dirichlet when a Dirichlet or
+# Interface fine cell maps to it, else Active when an Active fine cell does. Stated
+# here over all fine cells at once (numpy scatter), independent of visiting order.
+
+@pytest.mark.parametrize("case", ["box32", "sphere64"])
+def test_topology_downsample_rule(case, box32d, sphere64):
+    dom, cfg, o = box32d if case == "box32" else sphere64
+    for l in range(cfg.vlevels - 1):
+        off_f, fl_f = o.level(l)
+        off_c, fl_c = o.level(l + 1)
+        x, y, z = (v.reshape(-1) for v in cell_coords(off_f))
+        f = fl_f.reshape(-1)
+        cx, cy, cz = (x + 1) // 2, (y + 1) // 2, (z + 1) // 2
+        shape = (cx.max() + 2, cy.max() + 2, cz.max() + 2)
+        hard = np.zeros(shape, dtype=bool)
+        act = np.zeros(shape, dtype=bool)
+        sel = (f & (domain.DIRICHLET | domain.INTERFACE)) != 0
+        hard[cx[sel], cy[sel], cz[sel]] = True
+        sel = (f & domain.ACTIVE) != 0
+        act[cx[sel], cy[sel], cz[sel]] = True
+        act &= ~hard
+        X, Y, Z = (v.reshape(-1) for v in cell_coords(off_c))
+        g = fl_c.reshape(-1)
+        inb = (X < shape[0]) & (Y < shape[1]) & (Z < shape[2])
+        exp_act = np.zeros(g.shape, dtype=bool)
+        exp_dir = np.zeros(g.shape, dtype=bool)
+        exp_act[inb] = act[X[inb], Y[inb], Z[inb]]
+        exp_dir[inb] = hard[X[inb], Y[inb], Z[inb]]
+        assert np.array_equal((g & domain.ACTIVE) != 0, exp_act), f"level {l + 1} Active"
+        assert np.array_equal((g & domain.DIRICHLET) != 0, exp_dir), f"level {l + 1} Dirichlet"
+        assert exp_act.sum() > 0 and exp_dir.sum() > 0
+        # every coarse cell that receives a fine Active, Dirichlet or Interface cell is on a page
+        assert int(act.sum()) == int(exp_act.sum()) and int(hard.sum()) == int(exp_dir.sum())
