@@ -477,13 +477,24 @@ hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n) {
     const SigmaDevice* G = S.sigma.get();
     int nsub = 0;
     for (auto& q : S.prob) nsub += q->sigma.nsub;
-    double v[16] = {S.setup_seconds, G ? G->factor_seconds : 0.0, G ? (double)G->plan_bytes : 0.0,
+    // fused heights of every triangular-solve plan, and how many use the wavefront-per-run form
+    double fused = 0, waved = 0;
+    if (G) {
+        auto count = [&](const TriPlan& T) {
+            for (size_t h = 0; h < T.fused.size(); ++h)
+                if (T.fused[h]) { fused += 1; waved += (h < T.wave.size() && T.wave[h]) ? 1 : 0; }
+        };
+        for (const SigmaLevel& L : G->lv) { count(L.Lii); count(L.Lrr); }
+        count(G->Lbottom);
+    }
+    double v[18] = {S.setup_seconds, G ? G->factor_seconds : 0.0, G ? (double)G->plan_bytes : 0.0,
                     (double)S.lv[0].host.npages(), (G && G->levels) ? (double)G->lv[0].n_if : (S.sigma_host().levels ? (double)S.sigma_host().n_if[0] : 0.0),
                     (G && G->levels) ? (double)G->lv[0].n_int : 0.0, (double)nsub,
                     G ? G->bottom_rho : -1.0, G ? (double)G->bottom_direct : 0.0, G ? (double)G->bottom_k : 0.0,
                     G ? G->bottom_seconds : 0.0, G ? (double)G->bottom_steps : 0.0, G ? G->bottom_check : -1.0,
-                    G ? (double)G->tuned_choices : 0.0, G ? (double)G->dense_top_rows : 0.0, S.bottom_local ? 1.0 : 0.0};
-    for (int i = 0; i < n && i < 16; ++i) out[i] = v[i];
+                    G ? (double)G->tuned_choices : 0.0, G ? (double)G->dense_top_rows : 0.0, S.bottom_local ? 1.0 : 0.0,
+                    fused, waved};
+    for (int i = 0; i < n && i < 18; ++i) out[i] = v[i];
     return HDD_OK;
 }
 

@@ -893,7 +893,13 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
     sn_begin = P.sn_begin; sl_begin = P.sl_begin; sl_rows = P.sl_rows; row_begin = P.row_begin; ci_begin = P.ci_begin; bi_begin = P.bi_begin;
     ci_entries = P.ci_entries; h_sn_list = P.sn_list; h_sz = P.sz; h_nr = P.nr;
     run_begin = P.run_begin; run_max = P.run_max;
-    wave.assign(P.H + 1, 0);
+    // HDD_FUSED_WAVE=1 / 0 forces the wavefront / workgroup form on every fused height (same
+    // results); unset, the autotuner picks per height
+    {
+        const char* e = std::getenv("HDD_FUSED_WAVE");
+        wave_forced = e && (std::string(e) == "0" || std::string(e) == "1");
+        wave.assign(P.H + 1, wave_forced && std::string(e) == "1" ? 1 : 0);
+    }
     run_q.upload(P.run_q); run_n.upload(P.run_n);
     max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
     orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
@@ -1160,7 +1166,7 @@ int TriPlan::tune(hipStream_t s) {
     for (int h = 0; h <= H; ++h) {
         if (fused[h]) {
             // a workgroup or a wavefront per run: the same bits, whichever is faster
-            if (run_begin[h + 1] == run_begin[h]) continue;
+            if (wave_forced || run_begin[h + 1] == run_begin[h]) continue;
             const float t_wg = time_height(h);
             wave[h] = 1;
             const float t_wave = time_height(h);

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <stdexcept>
 #include <omp.h>
@@ -68,8 +69,11 @@ void sym_eig(int n, std::vector<double>& A, std::vector<double>& w) {
         for (;;) {
             int m = l;
             for (; m + 1 < n; ++m) {
+                // negligible: below rounding of the diagonal pair (a threshold under one ulp, as
+                // 1e-16 was, can stall when the spectrum spans many decades, as it does for a
+                // nearly singular bottom system of a pure-Neumann domain)
                 const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= 1e-16 * dd) break;
+                if (std::fabs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
             }
             if (m == l) break;
             if (++iter > 100) throw std::runtime_error("sym_eig: QL iteration did not converge");

@@ -332,14 +332,14 @@ def empty_subdomain_box(n: int = 48, sub: int = 16, vlevels: int = 3, seed: int 
     return d, DDConfig(sub_size=sub, vlevels=vlevels)
 
 
-def batch_problems(count: int = 64, seed: int = 0, sub: int = 32, vlevels: int = 4):
-    """Uneven independent problems: edge n = 32 * round(exp(U[ln 64, ln 256]) / 32),
-    1-4 sphere obstacles of radius 0.1-0.3 n, a delta of 1.0 at a random Active
-    cell, sub_size 32 (enclosed pockets are dropped)."""
+def batch_problems(count: int = 64, seed: int = 0, sub: int = 32, vlevels: int = 4, nmin: int = 64, nmax: int = 256):
+    """Uneven independent problems: edge n = 32 * round(exp(U[ln nmin, ln nmax]) / 32)
+    (64-256 for the 64-problem workload), 1-4 sphere obstacles of radius 0.1-0.3 n,
+    a delta of 1.0 at a random Active cell, sub_size 32 (enclosed pockets are dropped)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
-        n = int(32 * round(float(np.exp(rng.uniform(np.log(64), np.log(256)))) / 32))
+        n = int(32 * round(float(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))) / 32))
         spheres = random_spheres(rng, n, int(rng.integers(1, 5)), 0.1, 0.3)
         flags = box_flags(n, spheres, drop_enclosed=True)
         cells = np.flatnonzero((flags & ACTIVE) != 0)

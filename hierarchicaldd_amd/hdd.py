@@ -329,7 +329,7 @@ class DDSolver:
     def stats(self):
         keys = ["setup_s", "factor_s", "sigma_device_bytes", "pages", "interface_dofs", "interior_dofs", "subdomains",
                 "bottom_rho", "bottom_direct", "bottom_modes", "bottom_eig_s", "bottom_krylov_steps", "bottom_check",
-                "tuned_launch_shapes", "dense_top_rows", "bottom_page_local"]
+                "tuned_launch_shapes", "dense_top_rows", "bottom_page_local", "fused_heights", "wave_heights"]
         out = np.zeros(len(keys), dtype=np.float64)
         self._check(lib().hdd_stats(self.h, _ptr(out), len(keys)))
         return dict(zip(keys, out.tolist()))

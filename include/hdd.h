@@ -316,8 +316,12 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * by default), out[15] 1 if the interior V-cycle's 200 coarsest-level sweeps
  * run page by page in one launch (bit-identical to 200 launches; set-up picks
  * it when that level's Active cells couple only within their pages,
- * HDD_BOTTOM_LOCAL=0 turns it off), else 0.
- * n = number of slots provided (up to 16). No reference counterpart. */
+ * HDD_BOTTOM_LOCAL=0 turns it off), else 0, out[16] tree heights of the
+ * triangular-solve plans done by one fused launch per height (a workgroup or
+ * a wavefront per run of supernodes), out[17] how many of those use the
+ * wavefront form (the autotuner picks it where it is faster; results are the
+ * same; HDD_FUSED_WAVE=1 / 0 in the environment forces one form).
+ * n = number of slots provided (up to 18). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
 /* Measurement: runs `reps` exact solves of one triangular-solve plan of sigma

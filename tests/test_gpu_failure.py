@@ -7,9 +7,15 @@ of the reference dd_pcg) on the same input:
   after max_iterations, DDS/CONJUGATE_GRADIENT_DUMPING.cpp:188-191,207-211);
 * a pure-Neumann domain (no Dirichlet cell anywhere, so A 1 = 0) with a
   right-hand side in the range of A: both converge, iteration counts within
-  +-1, histories within the single-precision bar of tests/test_gpu_pcg.py;
+  +-1, and the GPU's x solves the system: max|b - A x| (float64, from the
+  flags) within 10x the tolerance. The residual histories are not compared:
+  the bottom of the sigma V-cycle keeps the near-null constant mode
+  (eigenvalue 1 - 7e-6 of its smoother), which the reference's 200 float
+  sweeps and the closed-form bottom carry with different rounding, and the
+  PCG iterates drift along the null space differently;
 * a subdomain with no Active cell (the reference skips it,
-  IFS/SIGMA_SMOOTHER.cpp:308): same bars.
+  IFS/SIGMA_SMOOTHER.cpp:308): iteration counts within +-1, histories within
+  1e-6 of max|r0| (the default-mode bar of tests/test_gpu_pcg.py).
 
 A domain with no interface at all (edge = sub_size) runs in
 tests/test_gpu_batch.py.
@@ -18,10 +24,12 @@ import numpy as np
 import pytest
 
 from hierarchicaldd_amd import domain
+from tests.reference_ops import face_residual
 
 pytestmark = pytest.mark.gpu
 
-HISTORY_BAR_SINGLE = 5e-4
+HISTORY_BAR_SINGLE = 5e-4       # relative per iteration, for the first iterations of a solve stopped early
+HISTORY_BAR_NORMALISED = 1e-6    # |h_gpu - h_oracle| / max|r0| over a whole solve
 HDD_OK, HDD_ERR_NOT_CONVERGED = 0, 4
 
 
@@ -73,8 +81,14 @@ def test_degenerate_domain(name, oracle_lib):
     finally:
         g.close()
     rel = _history_rel(h_g, h_o)
-    print(f"[{name}] oracle iters {it_o} gpu iters {rep.iterations} history max rel diff {rel:.3e}")
+    n = min(len(h_g), len(h_o))
+    nrm = float(np.abs(h_g[:n] - h_o[:n]).max() / h_o[0])
+    true_res = float(np.abs(face_residual(dom.page_offsets, dom.flags, x.astype(np.float64), dom.b.astype(np.float64))).max())
+    print(f"[{name}] oracle iters {it_o} gpu iters {rep.iterations} history max rel diff {rel:.3e}, normalised by max|r0| {nrm:.3e}, "
+          f"true residual {true_res / tol:.2f} tol")
     assert st == HDD_OK and rep.converged == 1
     assert it_o > 0 and abs(rep.iterations - it_o) <= 1
     assert np.isfinite(x).all()
-    assert rel < HISTORY_BAR_SINGLE
+    assert true_res <= 10 * tol
+    if not name.startswith("neumann"):
+        assert nrm < HISTORY_BAR_NORMALISED

@@ -22,6 +22,8 @@ CASES = {
     "box32_sub16": lambda: domain.small_box(32, 16, 3, seed=1),
     "sphere64_sub16": domain.config0,
     "box48_sub16": lambda: domain.small_box(48, 16, 3, seed=2),
+    # the 512^3 sparse workload's geometry (zero Dirichlet outside a union of spheres, N(0,1) rhs) at 96^3
+    "sparse96_sub16": lambda: domain.sparse_spheres(96, 6, 6, 18, seed=2, sub=16, vlevels=3),
 }
 
 

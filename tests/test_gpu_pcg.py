@@ -1,19 +1,22 @@
 """End-to-end DD-PCG on the GPU against the oracle's CPU dd_pcg.
 
-Two modes of the interface solver:
+Two accuracy modes of the interface solver, each with its bar:
 
-* exact-solve parity (one step of iterative refinement after every sigma
-  solve): the GPU's exact solves then agree with the oracle's double-precision
-  ones to float rounding, and the bar is the one for the whole path: iteration
-  count within +-1 of the oracle's and the residual history (max|r| per
-  iteration) within 1e-5 relative over the common prefix;
-* the default mode (FP32 factors, one solve each, the precision of the
-  reference's single-precision PARDISO): iteration count within +-1 and the
-  history within 5e-4. Single-precision sigma solves move the history by up
-  to ~1.3e-4 from exact ones on the 64^3 workload whatever implements them
-  (tests/test_oracle_pinning.py::test_single_precision_sigma_solves_move_histories
-  measures it on the oracle itself), so 1e-5 is not a bar any single-precision
-  solver meets.
+* reference-exact mode: every sigma solve followed by one step of iterative
+  refinement (exact solves, as the oracle's double-precision factors give) and
+  the bottom of the sigma V-cycle run as the reference's 200 literal smooths
+  (HDD_SIGMA_BOTTOM=loop). The residual history (max|r| per iteration) must
+  match the oracle's within 1e-5 relative at every iteration, and the
+  iteration count exactly; measured: identical to the last bit on every case.
+* the default (fast) mode: FP32 factors, one solve each (the precision of the
+  reference's single-precision PARDISO) and the bottom as one exact solve plus
+  a low-rank correction (its 200 smooths evaluated in closed form). Iteration
+  count within +-1, and the history within 1e-6 of the initial max|r| at every
+  iteration. A per-iteration relative bar does not apply to single-precision
+  solves: rounding the oracle's own sigma solves to float moves its late
+  iterations by up to 41% on the sub_size-32 cases
+  (tests/test_oracle_pinning.py::test_single_precision_sigma_solves_move_histories),
+  while staying within 1e-6 of max|r0|.
 
 Both stop at max|r| <= 1e-6 max|b|. Cases include the 64^3 workload itself and
 the 256^3 workload's DD parameters (sub_size 32, vlevels 4) at 64^3 and 96^3.
@@ -50,8 +53,8 @@ CASES = {
     "sphere96_sub32": lambda: domain.sphere_case(96, 32, 4, delta=(32, 32, 32)),
 }
 
-HISTORY_BAR_EXACT = 1e-5
-HISTORY_BAR_SINGLE = 5e-4
+HISTORY_BAR_EXACT = 1e-5         # reference-exact mode, relative per iteration
+HISTORY_BAR_NORMALISED = 1e-6    # default mode, |h_gpu - h_oracle| / max|r0|
 
 
 @pytest.fixture(scope="module", params=list(CASES))
@@ -60,6 +63,7 @@ def pair(request, oracle_lib):
     dom, cfg = CASES[request.param]()
     print(f"[case {request.param}] pages={dom.n_pages}", flush=True)
     o = oracle_lib.Oracle(dom, cfg)
+    o.solution = o.solve(1e-6 * float(np.abs(dom.b).max()), 200)[:2]  # (iterations, history), solved once
     g = hdd.DDSolver(dom, cfg)
     yield request.param, dom, cfg, o, g
     g.close()
@@ -68,6 +72,11 @@ def pair(request, oracle_lib):
 def _history_rel(h_g, h_o):
     n = min(len(h_o), len(h_g))
     return float((np.abs(h_g[:n] - h_o[:n]) / np.maximum(h_o[:n], 1e-30)).max())
+
+
+def _history_normalised(h_g, h_o):
+    n = min(len(h_o), len(h_g))
+    return float(np.abs(h_g[:n] - h_o[:n]).max() / h_o[0])
 
 
 def test_apply_preconditioner(pair):
@@ -89,23 +98,40 @@ def test_apply_preconditioner(pair):
 
 
 def test_pcg_matches_oracle(pair):
+    """Default mode: iterations +-1, history within 1e-6 of max|r0|."""
     name, dom, cfg, o, g = pair
     tol = 1e-6 * float(np.abs(dom.b).max())
-    it_o, h_o, _ = o.solve(tol, 200)
+    it_o, h_o = o.solution
     it_g, h_g, secs = g.solve(tol, 200)
-    rel = _history_rel(h_g, h_o)
-    print(f"[{name}] oracle iters {it_o} gpu iters {it_g} gpu seconds {secs:.4f} history max rel diff {rel:.3e}")
+    rel, nrm = _history_rel(h_g, h_o), _history_normalised(h_g, h_o)
+    print(f"[{name}] oracle iters {it_o} gpu iters {it_g} gpu seconds {secs:.4f} history max rel diff {rel:.3e}, "
+          f"normalised by max|r0| {nrm:.3e}")
     assert it_o > 0 and it_g > 0
     assert abs(it_g - it_o) <= 1
     assert h_g[-1] <= tol
-    assert rel < HISTORY_BAR_SINGLE
-    g.set_sigma_refinement(1)
+    assert nrm < HISTORY_BAR_NORMALISED
+
+
+def test_pcg_reference_exact_mode(pair):
+    """Exact sigma solves and the reference's 200 literal bottom smooths: the oracle's history."""
+    import os
+    from hierarchicaldd_amd import hdd
+    name, dom, cfg, o, _ = pair
+    tol = 1e-6 * float(np.abs(dom.b).max())
+    it_o, h_o = o.solution
+    os.environ["HDD_SIGMA_BOTTOM"] = "loop"
     try:
+        g = hdd.DDSolver(dom, cfg)
+    finally:
+        os.environ.pop("HDD_SIGMA_BOTTOM", None)
+    try:
+        g.set_sigma_refinement(1)
         it_r, h_r, secs_r = g.solve(tol, 200)
     finally:
-        g.set_sigma_refinement(0)
+        g.close()
     rel_r = _history_rel(h_r, h_o)
-    print(f"[{name}] refined: gpu iters {it_r} seconds {secs_r:.4f} history max rel diff {rel_r:.3e}")
-    assert it_r > 0 and abs(it_r - it_o) <= 1
+    print(f"[{name}] reference-exact mode: gpu iters {it_r} seconds {secs_r:.4f} history max rel diff {rel_r:.3e}, "
+          f"bit-identical {np.array_equal(h_r, h_o)}")
+    assert it_r == it_o
     assert h_r[-1] <= tol
     assert rel_r < HISTORY_BAR_EXACT

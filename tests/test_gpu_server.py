@@ -99,7 +99,8 @@ def test_server_mixed_schedules():
 
 
 def test_server_continuous_batching_and_cancel():
-    """Two requests that never converge (tolerance 0) hold both device slots; probe
+    """Two requests that never converge (a negative tolerance: the recursive residual of a
+    float PCG keeps falling and would pass any tolerance >= 0) hold both device slots; probe
     requests submitted meanwhile queue, and one cancelled while queued never runs.
     Then one long request is cancelled mid-flight: the probes join the other one
     mid-solve (concurrent units >= 2) and each gives the bits of its solve alone."""
@@ -115,8 +116,8 @@ def test_server_continuous_batching_and_cancel():
         g.close()
     srv = hdd.Server(setup_threads=2, groups=2, max_batch=1)
     try:
-        long_a = srv.submit(long_dom, long_cfg, 0.0, 1000000)
-        long_b = srv.submit(long_dom, long_cfg, 0.0, 1000000)
+        long_a = srv.submit(long_dom, long_cfg, -1.0, 1000000)
+        long_b = srv.submit(long_dom, long_cfg, -1.0, 1000000)
         time.sleep(2.0)  # both long requests set up and iterating
         queued = srv.submit(*probes[0], _tol(probes[0][0]), 300)
         assert srv.cancel(queued)
