@@ -313,6 +313,20 @@ hdd_status hdd_inner_product(hdd_ctx* ctx, int ch_a, int ch_b, double* out) {
     HDD_TRY(ctx, ctx->solver->inner_product(ch_a, ch_b, out));
 }
 
+hdd_status hdd_inner_product_noalloc(const float* a, const float* b, int64_t n_pages, double* out, void* stream) {
+    if (!a || !b || !out || n_pages < 1 || n_pages > HDD_NOALLOC_MAX_PAGES) {
+        g_create_error = "hdd_inner_product_noalloc: null pointer, or n_pages outside [1, HDD_NOALLOC_MAX_PAGES]";
+        return HDD_ERR_ARG;
+    }
+    try {
+        launch_inner_product_noalloc(a, b, n_pages, out, static_cast<hipStream_t>(stream));
+        return HDD_OK;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return HDD_ERR_DEVICE;
+    }
+}
+
 hdd_status hdd_convergence_norm(hdd_ctx* ctx, int ch, float* out) {
     if (!dev_ctx(ctx) || !ch_ok(ch) || !out) return HDD_ERR_ARG;
     HDD_TRY(ctx, ctx->solver->convergence_norm(ch, out));

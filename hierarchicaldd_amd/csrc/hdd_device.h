@@ -62,6 +62,8 @@ void launch_vector_op(int op, float* out, const float* a, const float* b, float 
 void launch_project(const DevLevel& L, float* v, hipStream_t s);
 void launch_sub_project(const DevLevel& L, const float* b, const float* q, float* r, hipStream_t s);
 int reduce_parts(size_t n);
+// hdd_inner_product_noalloc: partial sums in the library's file-scope scratch array, no allocation
+void launch_inner_product_noalloc(const float* a, const float* b, int64_t n_pages, double* out, hipStream_t s);
 
 // Per-problem view of the level-0 cells of a batch (see the PCG kernels in hdd_kernels.hip).
 struct DevSegments {

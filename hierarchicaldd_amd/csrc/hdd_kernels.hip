@@ -567,6 +567,42 @@ __global__ void __launch_bounds__(256) k_unscale_project(DevLevel L, const __hal
 }
 
 // ---------------------------------------------------------------------------
+// Inner product that allocates nothing (hdd_inner_product_noalloc): the block
+// partials of k_seg_dot_partial for one problem of n_pages pages go to the
+// file-scope array `scratch` (one copy per device, held until the process
+// ends), and the final sum of k_seg_finish (mode 3) to the caller's device
+// double. The blocks stride over the cells, and both reductions add, exactly
+// as those kernels do for a one-problem context, so the value has the bits of
+// hdd_inner_product on a context holding the same pages.
+constexpr int kNoAllocParts = 1024;  // reduce_parts never exceeds it
+static __device__ __align__(256) double scratch[kNoAllocParts];
+
+__global__ void __launch_bounds__(256) k_dot_partial_flat(const float* __restrict__ a, const float* __restrict__ b, size_t n, int g,
+                                                          double* __restrict__ part) {
+    const int c = blockIdx.x;
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (size_t i = (size_t)c * 256 + threadIdx.x; i < n; i += (size_t)g * 256) s += (double)(a[i] * b[i]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void __launch_bounds__(256) k_dot_finish_flat(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+    __shared__ double sd[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) s += part[i];
+    sd[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sd[threadIdx.x] += sd[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sd[0];
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 static inline int grid_for(size_t n) { size_t g = (n + 255) / 256; return (int)(g > 4096 ? 4096 : (g == 0 ? 1 : g)); }
 
@@ -665,6 +701,36 @@ void launch_sub_project(const DevLevel& L, const float* b, const float* q, float
     if (L.np) hipLaunchKernelGGL(k_sub_project, dim3(L.np), dim3(256), 0, s, L, b, q, r);
 }
 int reduce_parts(size_t n) { return grid_for(n) > 1024 ? 1024 : grid_for(n); }
+
+// Device address of `scratch` on the current device: hipGetSymbolAddress once
+// per device, with that device current, kept for later calls (each device
+// holds its own copy of the array at its own address; the array's name in
+// host code is not a device address).
+static double* noalloc_scratch() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* ptr[kMaxDevices] = {};
+    int dev = 0;
+    hdd_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("inner product (no allocation): device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        hdd_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
+void launch_inner_product_noalloc(const float* a, const float* b, int64_t n_pages, double* out, hipStream_t s) {
+    const size_t n = (size_t)n_pages * kPageCells;
+    const int g = reduce_parts((size_t)(n_pages + 1) * kPageCells);  // the partition of a context of n_pages pages
+    if (g > kNoAllocParts) throw std::runtime_error("inner product (no allocation): more partial sums than the scratch holds");
+    double* part = noalloc_scratch();
+    hipLaunchKernelGGL(k_dot_partial_flat, dim3(g), dim3(256), 0, s, a, b, n, g, part);
+    hipLaunchKernelGGL(k_dot_finish_flat, dim3(1), dim3(256), 0, s, (const double*)part, g, out);
+    hdd_check(hipGetLastError(), "inner product (no allocation)");
+}
 void launch_seg_copy2(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_copy2, dim3(grid_for(S.ncells)), dim3(256), 0, s, out, in1, in2, (const float*)S.scal, slot, S.page_prob,
                        S.active, S.ncells);

@@ -225,6 +225,22 @@ hdd_status hdd_multiply(hdd_ctx* ctx, int ch_x, int ch_result);
  * Replaces CG_SYSTEM::Inner_Product (DDS/CG_SYSTEM.cpp:50-73). */
 hdd_status hdd_inner_product(hdd_ctx* ctx, int ch_a, int ch_b, double* out);
 
+/* The same inner product with no allocation and no context. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). a and b are device arrays of n_pages*256 floats (the pages of a
+ * level-0 channel in page order); *out (device memory) receives the sum over
+ * every cell of (float)(a*b), summed in double, with the bits hdd_inner_product
+ * gives on a one-problem context holding the same pages. Enqueued on `stream`
+ * (a hipStream_t; NULL: the null stream) and returns before it runs. The
+ * partial sums live in a file-scope device array of 1024 doubles (8 KiB), one
+ * per device, held until the process ends; nothing is allocated or freed.
+ * Calls that overlap on one device race on that array. n_pages must be in
+ * [1, HDD_NOALLOC_MAX_PAGES]; hdd_last_error(NULL) gives the message of a
+ * failure. */
+#define HDD_NOALLOC_MAX_PAGES (1LL << 24)
+hdd_status hdd_inner_product_noalloc(const float* a, const float* b, int64_t n_pages, double* out, void* stream);
+
 /* out[p] = max |v| over the cells of problem p. Replaces
  * CG_SYSTEM::Convergence_Norm (DDS/CG_SYSTEM.cpp:78-96). */
 hdd_status hdd_convergence_norm(hdd_ctx* ctx, int ch, float* out);
