@@ -93,7 +93,8 @@ def main():
     solver = hdd.DDSolver(dom, cfg)
     st = solver.stats()
     log(f"[bench] setup {time.time() - t0:.1f}s (factorisation {st['factor_s']:.1f}s, interface-solver plans "
-        f"{st['sigma_device_bytes'] / 2**30:.2f} GiB, bottom rho {st['bottom_rho']:.4f}, direct bottom {bool(st['bottom_direct'])})")
+        f"{st['sigma_device_bytes'] / 2**30:.2f} GiB, bottom rho {st['bottom_rho']:.4f}, direct bottom {bool(st['bottom_direct'])}, "
+        f"fused heights {st['fused_heights']:.0f} of which {st['wave_heights']:.0f} a wavefront per run)")
     tol = 1e-6 * float(np.abs(dom.b).max())
 
     if args.plan_only:

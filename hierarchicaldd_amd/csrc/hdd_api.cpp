@@ -179,6 +179,12 @@ hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16) {
     HDD_TRY(ctx, ctx->solver->set_half_storage(fp16 != 0));
 }
 
+hdd_status hdd_set_dot_order(hdd_ctx* ctx, int serial) {
+    if (!ctx) return HDD_ERR_ARG;
+    ctx->solver->serial_dots = serial != 0;
+    return HDD_OK;
+}
+
 hdd_status hdd_set_sigma_refinement(hdd_ctx* ctx, int steps) {
     if (!ctx) return HDD_ERR_ARG;
     if (ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }

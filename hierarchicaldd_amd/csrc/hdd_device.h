@@ -77,6 +77,7 @@ struct DevSegments {
     float* fpart = nullptr;           // [nprob*max_parts]
     float* scal = nullptr;            // [nprob*8]
     double* dscal = nullptr;          // [nprob*8]
+    bool serial_dots = false;         // inner products in the reference's serial order (k_seg_dot_serial)
 };
 void launch_seg_copy2(const DevSegments& S, float* out, const float* in1, const float* in2, int slot, hipStream_t s);
 void launch_seg_xr_update(const DevSegments& S, float* x, float* r, const float* sv, const float* q, hipStream_t s);

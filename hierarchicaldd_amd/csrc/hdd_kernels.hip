@@ -473,6 +473,29 @@ __global__ void __launch_bounds__(256) k_seg_dot_partial(const float* __restrict
     __syncthreads();
     if (threadIdx.x == 0) part[(size_t)p * max_parts + c] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
+// The reference's order of CG_SYSTEM::Inner_Product (DDS/CG_SYSTEM.cpp:62-72): one
+// running double sum over the problem's pages in page order and their cells in
+// cell order, each term the float product. One workgroup per problem stages 1024
+// products at a time in LDS and one lane adds them in sequence; the sum goes to
+// the problem's first partial and its other partials are zero, so k_seg_finish
+// gives it unchanged. For checking against the reference's rounding, not speed.
+__global__ void __launch_bounds__(256) k_seg_dot_serial(const float* __restrict__ a, const float* __restrict__ b, double* __restrict__ part,
+                                                        const int* __restrict__ pg0, const int* __restrict__ parts, int max_parts) {
+    const int p = blockIdx.x;
+    const size_t base = (size_t)pg0[p] * 256, n = (size_t)(pg0[p + 1] - pg0[p]) * 256;
+    __shared__ double buf[1024];
+    double s = 0.0;
+    for (size_t c0 = 0; c0 < n; c0 += 1024) {
+        const size_t cnt = n - c0 < 1024 ? n - c0 : 1024;
+        for (size_t t = threadIdx.x; t < cnt; t += 256) buf[t] = (double)(a[base + c0 + t] * b[base + c0 + t]);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (size_t t = 0; t < cnt; ++t) s += buf[t];
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < parts[p]; c += 256) part[(size_t)p * max_parts + c] = c == 0 ? s : 0.0;
+}
+
 __global__ void __launch_bounds__(256) k_seg_maxabs_partial(const float* __restrict__ a, float* __restrict__ part,
                                                             const int* __restrict__ pg0, const int* __restrict__ parts, int max_parts) {
     const int p = blockIdx.y, c = blockIdx.x, g = parts[p];
@@ -740,7 +763,10 @@ void launch_seg_xr_update(const DevSegments& S, float* x, float* r, const float*
                        S.active, S.ncells);
 }
 void launch_seg_dot(const DevSegments& S, const float* a, const float* b, int mode, hipStream_t s) {
-    hipLaunchKernelGGL(k_seg_dot_partial, dim3(S.max_parts, S.nprob), dim3(256), 0, s, a, b, S.dpart, S.pg0, S.parts, S.max_parts);
+    if (S.serial_dots)
+        hipLaunchKernelGGL(k_seg_dot_serial, dim3(S.nprob), dim3(256), 0, s, a, b, S.dpart, S.pg0, S.parts, S.max_parts);
+    else
+        hipLaunchKernelGGL(k_seg_dot_partial, dim3(S.max_parts, S.nprob), dim3(256), 0, s, a, b, S.dpart, S.pg0, S.parts, S.max_parts);
     hipLaunchKernelGGL(k_seg_finish, dim3(S.nprob), dim3(256), 0, s, (const double*)S.dpart, (const float*)nullptr, S.parts, S.max_parts,
                        mode, S.active, S.scal, S.dscal);
 }

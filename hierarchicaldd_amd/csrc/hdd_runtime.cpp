@@ -519,6 +519,7 @@ DevSegments Solver::segments() {
     sg.nprob = nprob(); sg.max_parts = max_parts; sg.ncells = (size_t)lv[0].host.npages() * kPageCells;
     sg.pg0 = seg_pg0.p; sg.parts = seg_parts.p; sg.page_prob = page_prob.p; sg.active = active.p;
     sg.dpart = dpart.p; sg.fpart = fpart.p; sg.scal = scal.p; sg.dscal = dscal.p;
+    sg.serial_dots = serial_dots;
     return sg;
 }
 

@@ -203,6 +203,7 @@ public:
     // float copy when the problem stops), and the true residual b - L x of the
     // double iterate replacing r every replace_every iterations and whenever the
     // recursive residual passes the test, which is then taken on the true one
+    bool serial_dots = false;          // inner products summed in the reference's serial order (slow; for parity checks)
     bool outer_fp64 = false;
     int replace_every = 25;
     DBuf<double> xd;

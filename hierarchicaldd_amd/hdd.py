@@ -59,7 +59,7 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_set_sigma_refinement", "hdd_sigma_factor_solve", "hdd_set_stream", "hdd_synchronize", "hdd_multiply",
            "hdd_inner_product", "hdd_convergence_norm", "hdd_project", "hdd_vector_copy", "hdd_vector_copy2",
            "hdd_vector_plus_equal", "hdd_vector_minus_equal", "hdd_vector_scale", "hdd_apply_preconditioner_channels",
-           "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_inner_product_noalloc"]
+           "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_set_dot_order", "hdd_inner_product_noalloc"]
 
 
 def lib():
@@ -132,6 +132,7 @@ def lib():
         L.hdd_apply_preconditioner_channels.argtypes = [P, I, I, I]
         L.hdd_set_pcg_precision.argtypes = [P, I, I]
         L.hdd_get_x_f64.argtypes = [P, I, P]
+        L.hdd_set_dot_order.argtypes = [P, I]
         L.hdd_inner_product_noalloc.argtypes = [P, P, C.c_int64, P, P]
         for name in EXPORTS:
             if name not in ("hdd_last_error", "hdd_server_last_error"):
@@ -291,6 +292,10 @@ class DDSolver:
         out = np.zeros((self.n_pages, 256), dtype=np.float64)
         self._check(lib().hdd_get_x_f64(self.h, problem, _ptr(out)))
         return out
+
+    def set_serial_dots(self, enable: bool):
+        """Inner products in the reference's serial summation order (slow; for parity checks)."""
+        self._check(lib().hdd_set_dot_order(self.h, int(bool(enable))))
 
     def set_sigma_refinement(self, steps: int):
         """Iterative-refinement steps after every exact solve of the interface solver (0: none, default)."""

@@ -139,6 +139,16 @@ hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16);
  * rounding of their result (each step costs one more solve). */
 hdd_status hdd_set_sigma_refinement(hdd_ctx* ctx, int steps);
 
+/* Summation order of the inner products of the PCG loop and hdd_inner_product:
+ * serial = 0 (default) sums each problem's products in double over a fixed
+ * tree of blocks (deterministic, full bandwidth); serial = 1 sums them in one
+ * running double sum in page order and cell order, the order of the
+ * reference's CG_SYSTEM::Inner_Product loop (DDS/CG_SYSTEM.cpp:62-72), so the
+ * float values of rho and s.q round exactly as the reference's do (one
+ * workgroup per problem: milliseconds per product on large grids; for parity
+ * checks against the reference's own rounding). */
+hdd_status hdd_set_dot_order(hdd_ctx* ctx, int serial);
+
 /* Precision of the PCG loop (no reference counterpart): fp64 = 0 (default)
  * keeps CONJUGATE_GRADIENT_DUMPING's float rho, s.q, alpha and beta
  * (DDS/CONJUGATE_GRADIENT_DUMPING.cpp:193-205), with its true residual only at

@@ -14,6 +14,10 @@ random r (the same generator as tests/gpu_common.random_rhs, seed 11), as
 checksums and the same sample of cells.
 
 Usage: OMP_NUM_THREADS=8 python tests/golden/make_sphere256.py
+       OMP_NUM_THREADS=8 python tests/golden/make_sphere256.py --single
+The second form runs the oracle with single-precision sigma solves (the
+rounding of the reference's single-precision PARDISO, iparm[27] = 1) and writes
+its iteration count and history to tests/golden/sphere256_sub32_single.json.
 """
 import json
 import os
@@ -43,6 +47,27 @@ def sample_cells(flags, seed=SAMPLE_SEED, count=SAMPLE_COUNT):
 def checksums(v):
     v = np.asarray(v, dtype=np.float64).reshape(-1)
     return {"sum": float(v.sum()), "sum_sq": float((v * v).sum()), "max_abs": float(np.abs(v).max())}
+
+
+def main_single():
+    from hierarchicaldd_amd import domain
+    from oracle import pyoracle
+    dom, cfg = domain.config1()
+    t0 = time.time()
+    o = pyoracle.Oracle(dom, cfg, sigma_single=True)
+    t_setup = time.time() - t0
+    print(f"oracle (single-precision sigma solves) setup {t_setup:.1f}s", flush=True)
+    tol = 1e-6 * float(np.abs(dom.b).max())
+    it, hist, secs = o.solve(tol, 500)
+    print(f"oracle solve: {it} iterations, {secs:.1f}s", flush=True)
+    out = {"workload": "sphere256_sub32",
+           "producer": "tests/golden/make_sphere256.py --single (oracle, single-precision sigma solves)",
+           "n_pages": int(dom.n_pages), "tolerance": tol, "iterations": int(it), "history": [float(h) for h in hist],
+           "oracle_seconds": {"setup": t_setup, "solve": secs, "threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0)}}
+    path = OUT.replace(".json", "_single.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=0)
+    print("wrote", path)
 
 
 def main():
@@ -87,4 +112,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if "--single" in sys.argv[1:]:
+        main_single()
+    else:
+        main()

@@ -127,11 +127,14 @@ def test_pcg_reference_exact_mode(pair):
     try:
         g.set_sigma_refinement(1)
         it_r, h_r, secs_r = g.solve(tol, 200)
+        # the inner products in the reference's serial summation order change nothing here
+        g.set_serial_dots(True)
+        it_s, h_s, _ = g.solve(tol, 200)
     finally:
         g.close()
-    rel_r = _history_rel(h_r, h_o)
+    rel_r, rel_s = _history_rel(h_r, h_o), _history_rel(h_s, h_o)
     print(f"[{name}] reference-exact mode: gpu iters {it_r} seconds {secs_r:.4f} history max rel diff {rel_r:.3e}, "
-          f"bit-identical {np.array_equal(h_r, h_o)}")
-    assert it_r == it_o
+          f"bit-identical {np.array_equal(h_r, h_o)}; serial-order inner products: {it_s} iters, max rel diff {rel_s:.3e}")
+    assert it_r == it_o and it_s == it_o
     assert h_r[-1] <= tol
-    assert rel_r < HISTORY_BAR_EXACT
+    assert rel_r < HISTORY_BAR_EXACT and rel_s < HISTORY_BAR_EXACT
