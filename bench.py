@@ -21,9 +21,12 @@ solve), set beside the GPU's rate on the same 64^3 problem.
 The other two parts of the metric are added to the same line:
   "batched": uneven independent problems (edge 64-256, 1-4 sphere obstacles,
       delta right-hand side, sub_size 32; --batch-count of them, 16 by default,
-      64 for the full set) solved one at a time on the GPU and then all at once
-      in one batched context; solves/s of both (solve phase only: host setup is
-      the same for both and is reported separately) and each batched result's
+      64 for the full set) solved one at a time on the GPU, then all at once in
+      three forms: every problem's own context iterating at once on its own
+      stream, one batched context (every stage one launch for all problems),
+      and four batched contexts of similar-size problems iterating at once;
+      solves/s of each (solve phase only: host setup is the same for all and is
+      reported separately), the best form's speedup, and each batched result's
       difference from its solve alone;
   "served": a request stream (edge log-uniform 32-384, up to 40% blocked by
       spheres, N(0,1) right-hand side, tolerance 1e-6) with Poisson arrivals at
@@ -54,6 +57,24 @@ METRIC = "256³ time-to-1e-6 rel-res; batched solves/s; p50/p99 served latency"
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def memory_in_use():
+    """Device memory in use (hipMemGetInfo) and this process's resident host memory, in GiB."""
+    import ctypes as C
+    free, total = C.c_size_t(), C.c_size_t()
+    dev = float("nan")
+    try:
+        if C.CDLL("libamdhip64.so").hipMemGetInfo(C.byref(free), C.byref(total)) == 0:
+            dev = (total.value - free.value) / 2**30
+    except OSError:
+        pass
+    rss = float("nan")
+    with open("/proc/self/status") as f:
+        for line in f:
+            if line.startswith("VmRSS:"):
+                rss = int(line.split()[1]) / 2**20
+    return f"device {dev:.1f} GiB in use, host RSS {rss:.1f} GiB"
 
 
 def main():
@@ -393,28 +414,59 @@ def batched_leg(hdd, count, seed=0):
         plain_s += secs
         iters.append(it)
         xs.append(g.get(hdd.CH_X))
-    log(f"[batched] set-up of {count} problems alone: {setup_wall:.1f}s wall, four at a time")
+    log(f"[batched] set-up of {count} problems alone: {setup_wall:.1f}s wall, four at a time; {memory_in_use()}")
     log(f"[batched] plain: {count} solves in {plain_s:.3f}s ({count / plain_s:.2f} solves/s), host setup {setup_s:.1f}s")
+    tol = [1e-6 * float(np.abs(d.b).max()) for d, _ in probs]
+    # the same contexts, every one iterating at once on its own stream (hdd_batches_pcg_solve)
+    hdd.solve_concurrently(singles, tol, 1000)  # warm-up
+    t0 = time.perf_counter()
+    res_c = hdd.solve_concurrently(singles, tol, 1000)
+    conc_s = time.perf_counter() - t0
+    log(f"[batched] {count} contexts iterating at once: {conc_s:.3f}s ({count / conc_s:.2f} solves/s, x {plain_s / conc_s:.2f} the plain path)")
+    # four batched contexts of problems of similar size iterating at once, then one batched context
+    # (every stage one launch for all problems); device memory holds at most two copies of the set
+    order = sorted(range(count), key=lambda p: probs[p][0].n_pages)
+    groups = [sorted(order[k * count // 4:(k + 1) * count // 4]) for k in range(4)] if count >= 8 else [list(range(count))]
     t0 = time.time()
-    bt = hdd.DDBatch(from_solvers=singles)
+    bts = [hdd.DDBatch(from_solvers=[singles[p] for p in grp]) for grp in groups]
     build_s = time.time() - t0
+    log(f"[batched] size-grouped batches built: {memory_in_use()}")
     for g in singles:
         g.close()
-    tol = [1e-6 * float(np.abs(d.b).max()) for d, _ in probs]
-    bt.solve_all(tol, 1000)  # warm-up
+    gtol = [tol[p] for grp in groups for p in grp]
+    hdd.solve_concurrently(bts, gtol, 1000)  # warm-up
     t0 = time.perf_counter()
-    res = bt.solve_all(tol, 1000)
+    res_g = hdd.solve_concurrently(bts, gtol, 1000)
+    grouped_s = time.perf_counter() - t0
+    git = [r[0] for r in res_g]
+    log(f"[batched] {len(bts)} size-grouped batches at once: {grouped_s:.3f}s ({count / grouped_s:.2f} solves/s, x {plain_s / grouped_s:.2f})")
+    bt = hdd.DDBatch(from_solvers=bts)
+    log(f"[batched] one batch built: {memory_in_use()}")
+    for b in bts:
+        b.close()
+    perm = [p for grp in groups for p in grp]  # problem order inside bt
+    bt.solve_all([tol[p] for p in perm], 1000)  # warm-up
+    t0 = time.perf_counter()
+    res = bt.solve_all([tol[p] for p in perm], 1000)
     batch_s = time.perf_counter() - t0
-    bit = [r[0] for r in res]
-    xdiff = max(float(np.abs(bt.x(p) - xs[p]).max() / max(np.abs(xs[p]).max(), 1e-30)) for p in range(count))
+    bit = [0] * count
+    for k, p in enumerate(perm):
+        bit[p] = res[k][0]
+    xdiff = max(float(np.abs(bt.x(k) - xs[p]).max() / max(np.abs(xs[p]).max(), 1e-30)) for k, p in enumerate(perm))
     bt.close()
-    log(f"[batched] batch: {count} solves in {batch_s:.3f}s ({count / batch_s:.2f} solves/s, x {plain_s / batch_s:.2f} the plain path), "
-        f"device build {build_s:.1f}s, iterations alone {iters} batched {bit}, max x rel diff {xdiff:.2e}")
-    return {"problems": count, "seed": seed, "plain_solves_per_s": count / plain_s, "batched_solves_per_s": count / batch_s,
-            "speedup": plain_s / batch_s, "plain_solve_s": plain_s, "batched_solve_s": batch_s, "host_setup_s": setup_s,
-            "setup_wall_s": setup_wall,
-            "batch_device_build_s": build_s, "max_iteration_diff": int(max(abs(a - b) for a, b in zip(iters, bit))),
-            "max_x_rel_diff_vs_alone": xdiff, "all_converged": bool(min(bit) >= 0)}
+    log(f"[batched] one batch: {count} solves in {batch_s:.3f}s ({count / batch_s:.2f} solves/s, x {plain_s / batch_s:.2f} the plain path), "
+        f"iterations alone {iters} batched {bit}, max x rel diff {xdiff:.2e}")
+    forms = {"one_batch": batch_s, "contexts_at_once": conc_s, "size_grouped_batches_at_once": grouped_s}
+    best = min(forms, key=forms.get)
+    it_c = [r[0] for r in res_c]
+    return {"problems": count, "seed": seed, "plain_solves_per_s": count / plain_s, "batched_solves_per_s": count / forms[best],
+            "batched_form": best, "speedup": plain_s / forms[best],
+            "forms_solves_per_s": {k: count / v for k, v in forms.items()},
+            "plain_solve_s": plain_s, "batched_solve_s": forms[best], "host_setup_s": setup_s, "setup_wall_s": setup_wall,
+            "batch_device_build_s": build_s,
+            "max_iteration_diff": int(max(abs(a - b) for a, b in zip(iters, bit))),
+            "contexts_at_once_iterations_equal": bool(it_c == iters),
+            "max_x_rel_diff_vs_alone": xdiff, "all_converged": bool(min(bit) >= 0 and min(it_c) >= 0 and min(git) >= 0)}
 
 
 def served_leg(hdd, n_req, rates, nmax, seed=1):

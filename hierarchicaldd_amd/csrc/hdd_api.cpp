@@ -161,6 +161,51 @@ hdd_status hdd_batch_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* prm, hdd_pcg_
     }
 }
 
+hdd_status hdd_batches_pcg_solve(hdd_ctx* const* ctxs, int count, const hdd_pcg_params* prm, hdd_pcg_report* rep) {
+    if (!ctxs || count < 1 || !prm) { g_create_error = "hdd_batches_pcg_solve: invalid arguments"; return HDD_ERR_ARG; }
+    std::vector<int> first(count + 1, 0);
+    for (int c = 0; c < count; ++c) {
+        if (!ctxs[c] || ctxs[c]->solver->host_only) { g_create_error = "hdd_batches_pcg_solve: null or host-only context"; return HDD_ERR_ARG; }
+        for (int d = 0; d < c; ++d)
+            if (ctxs[d] == ctxs[c]) { g_create_error = "hdd_batches_pcg_solve: a context is listed twice"; return HDD_ERR_ARG; }
+        first[c + 1] = first[c] + ctxs[c]->solver->nprob();
+    }
+    for (int p = 0; p < first[count]; ++p)
+        if (prm[p].max_iterations < 0) { g_create_error = "hdd_batches_pcg_solve: negative iteration limit"; return HDD_ERR_ARG; }
+    int c = 0;
+    try {
+        // every context's PCG advanced one iteration per round, each on its own stream, so their
+        // kernels overlap on the device (the serving loop's driver, for contexts built in advance)
+        std::vector<PcgRun> runs(count);
+        for (c = 0; c < count; ++c) {
+            std::vector<double> tol;
+            std::vector<int> mit;
+            for (int p = first[c]; p < first[c + 1]; ++p) { tol.push_back(prm[p].tolerance); mit.push_back(prm[p].max_iterations); }
+            ctxs[c]->solver->pcg_start(runs[c], tol, mit);
+        }
+        for (bool all = false; !all;) {
+            all = true;
+            for (c = 0; c < count; ++c)
+                if (!runs[c].done()) all = ctxs[c]->solver->pcg_advance(runs[c]) && all;
+        }
+        bool conv = true;
+        for (c = 0; c < count; ++c)
+            for (int p = first[c]; p < first[c + 1]; ++p) {
+                const PcgResult& r = runs[c].out[p - first[c]];
+                if (rep) {
+                    rep[p].iterations = r.iterations; rep[p].seconds = r.seconds;
+                    rep[p].final_norm = r.hist.empty() ? 0.0 : r.hist.back(); rep[p].converged = r.converged;
+                }
+                conv = conv && r.converged;
+            }
+        return conv ? HDD_OK : HDD_ERR_NOT_CONVERGED;
+    } catch (std::exception& e) {
+        if (c < count) ctxs[c]->err = e.what();
+        g_create_error = e.what();
+        return HDD_ERR_DEVICE;
+    }
+}
+
 hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host) {
     if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || !host || problem < 0 || problem >= ctx->solver->nprob()) return HDD_ERR_ARG;

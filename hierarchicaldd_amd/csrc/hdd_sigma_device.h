@@ -28,6 +28,10 @@ struct TriPlan {
     std::vector<int> sn_begin, sl_begin, sl_rows, row_begin, ci_begin, bi_begin, max_sz, big, fused, G, Gc, Gf, Gb, run_begin, run_max;
     std::vector<int> wave;  // per fused height: 1 = a wavefront per run (k_tri_*_wave), 0 = a workgroup per run (same results)
     bool wave_forced = false;  // HDD_FUSED_WAVE set: the tuner leaves `wave` alone
+    // per fused height: a cap on the workgroups of its launch, each looping over runs (default:
+    // no cap, a workgroup per run; the tuner tries caps; results do not depend on it)
+    std::vector<int> pgrid;
+    static constexpr int kFusedGroups = 1 << 30;
     DBuf<int> run_q, run_n, orig_of, s0, sz, nr, roff, srows, sn_of, sn_prob, sn_list, sl_sn, sl_r0, rowlist, inc_ptr, ci_sn, ci_a0, ci_a1, bi_sn, bi_c, bi_j0;
     DBuf<long long> loff, doff, cofs, cpos, bofs;
     DBuf<float> diag, diagT, l21, c, bpart, t, y;

@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
 // and each member's sums run in exactly the order of a one-member run.
 // forward: t = b - (incidence sums) -> y = Linv t (LDS) -> c_g = L21_g y
 template <int G, int M>
-__global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
+__device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
                                                        const long long* __restrict__ doff, const float* __restrict__ diag,
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
     __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
     __shared__ __align__(16) float ys[TriPlanHost::kSliceRows][M];
     __shared__ int base_s[M];
-    const int q0 = run_q[r0 + blockIdx.x], cnt = run_n[r0 + blockIdx.x];
+    const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
     bool any = false;
     for (int j = 0; j < cnt; ++j) any = any || !sn_stopped(sn_prob, active, sn_list[q0 + j]);
@@ -326,11 +326,27 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, const int* __rest
     }
 }
 
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                       const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                       const int* __restrict__ sz, const int* __restrict__ nr,
+                                                       const long long* __restrict__ doff, const float* __restrict__ diag,
+                                                       const int* __restrict__ inc_ptr, const long long* __restrict__ cpos,
+                                                       const int* __restrict__ orig_of, const float* __restrict__ b,
+                                                       const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                       const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
+                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    for (int run = blockIdx.x; run < nruns; run += gridDim.x) {
+        fwd_small_run<G, M>(run, r0, run_q, run_n, sn_list, s0, sz, nr, doff, diag, inc_ptr, cpos, orig_of, b, loff, cofs, l21, c, y, sn_prob, active);
+        __syncthreads();  // the next run reuses the LDS vectors
+    }
+}
+
 // backward: t_j = y_j - sum_a L21_g[a, j] z[rows_g[a]] (four wavefronts split
 // the rows: wavefront w takes rows a = w mod 4 in ascending order; lanes own
 // columns j and j + 64; LDS reduction) -> z = Linv^T t
 template <int G, int M>
-__global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
+__device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
                                                        const int* __restrict__ roff, const int* __restrict__ srows,
@@ -344,7 +360,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
     __shared__ float part[4][M][TriPlanHost::kSliceRows];
     __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
     __shared__ int base_s[M], roff_s[M];
-    const int q0 = run_q[r0 + blockIdx.x], cnt = run_n[r0 + blockIdx.x];
+    const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
     bool any = false;
     for (int j = 0; j < cnt; ++j) any = any || !sn_stopped(sn_prob, active, sn_list[q0 + j]);
@@ -422,6 +438,22 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, const int* __rest
     }
 }
 
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                       const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                       const int* __restrict__ sz, const int* __restrict__ nr,
+                                                       const int* __restrict__ roff, const int* __restrict__ srows,
+                                                       const long long* __restrict__ loff, const float* __restrict__ l21,
+                                                       const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                       const int* __restrict__ orig_of, const float* __restrict__ y,
+                                                       float* __restrict__ z, float* __restrict__ x,
+                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    for (int run = blockIdx.x; run < nruns; run += gridDim.x) {
+        bwd_small_run<G, M>(run, r0, run_q, run_n, sn_list, s0, sz, nr, roff, srows, loff, l21, doff, diagT, orig_of, y, z, x, sn_prob, active);
+        __syncthreads();  // the next run reuses the LDS vectors
+    }
+}
+
 // Wavefront-per-run forms of the two kernels above, four runs per 256-thread
 // workgroup: heights of many tiny supernodes leave most of a workgroup idle and
 // pay a workgroup barrier per phase, while a wavefront needs none. Every row's
@@ -436,7 +468,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 template <int G, int M>
-__global__ void __launch_bounds__(256) k_tri_fwd_wave(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+__device__ __forceinline__ void fwd_wave_run(int run, int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                       const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                       const int* __restrict__ sz, const int* __restrict__ nr,
                                                       const long long* __restrict__ doff, const float* __restrict__ diag,
@@ -449,8 +481,6 @@ __global__ void __launch_bounds__(256) k_tri_fwd_wave(int r0, int nruns, const i
     __shared__ __align__(16) float ys_all[4][TriPlanHost::kSliceRows][M];
     __shared__ int base_all[4][M];
     const int w = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
-    const int run = blockIdx.x * 4 + w;
-    if (run >= nruns) return;
     float(*ts)[M] = ts_all[w];
     float(*ys)[M] = ys_all[w];
     int* base_s = base_all[w];
@@ -534,7 +564,23 @@ __global__ void __launch_bounds__(256) k_tri_fwd_wave(int r0, int nruns, const i
 }
 
 template <int G, int M>
-__global__ void __launch_bounds__(256) k_tri_bwd_wave(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+__global__ void __launch_bounds__(256) k_tri_fwd_wave(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                      const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                      const int* __restrict__ sz, const int* __restrict__ nr,
+                                                      const long long* __restrict__ doff, const float* __restrict__ diag,
+                                                      const int* __restrict__ inc_ptr, const long long* __restrict__ cpos,
+                                                      const int* __restrict__ orig_of, const float* __restrict__ b,
+                                                      const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                      const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
+                                                      const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    for (int run = blockIdx.x * 4 + (int)(threadIdx.x >> 6); run < nruns; run += gridDim.x * 4) {
+        fwd_wave_run<G, M>(run, r0, nruns, run_q, run_n, sn_list, s0, sz, nr, doff, diag, inc_ptr, cpos, orig_of, b, loff, cofs, l21, c, y, sn_prob, active);
+        wave_sync();  // the next run reuses this wavefront's LDS vectors
+    }
+}
+
+template <int G, int M>
+__device__ __forceinline__ void bwd_wave_run(int run, int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                       const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                       const int* __restrict__ sz, const int* __restrict__ nr,
                                                       const int* __restrict__ roff, const int* __restrict__ srows,
@@ -548,8 +594,6 @@ __global__ void __launch_bounds__(256) k_tri_bwd_wave(int r0, int nruns, const i
     __shared__ __align__(16) float ts_all[4][TriPlanHost::kSliceRows][M];
     __shared__ int base_all[4][M], roff_all[4][M];
     const int w = threadIdx.x >> 6, lane64 = threadIdx.x & 63;
-    const int run = blockIdx.x * 4 + w;
-    if (run >= nruns) return;
     float(*zr)[M] = zr_all[w];
     float(*ts)[M] = ts_all[w];
     int* base_s = base_all[w];
@@ -629,6 +673,22 @@ __global__ void __launch_bounds__(256) k_tri_bwd_wave(int r0, int nruns, const i
                 const float v = group_sum<G>(acc[j]);
                 if (gl == 0) { z[base_s[j] + r] = v; x[orig_of[base_s[j] + r]] = v; }
             }
+    }
+}
+
+template <int G, int M>
+__global__ void __launch_bounds__(256) k_tri_bwd_wave(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
+                                                      const int* __restrict__ sn_list, const int* __restrict__ s0,
+                                                      const int* __restrict__ sz, const int* __restrict__ nr,
+                                                      const int* __restrict__ roff, const int* __restrict__ srows,
+                                                      const long long* __restrict__ loff, const float* __restrict__ l21,
+                                                      const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                      const int* __restrict__ orig_of, const float* __restrict__ y,
+                                                      float* __restrict__ z, float* __restrict__ x,
+                                                      const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    for (int run = blockIdx.x * 4 + (int)(threadIdx.x >> 6); run < nruns; run += gridDim.x * 4) {
+        bwd_wave_run<G, M>(run, r0, nruns, run_q, run_n, sn_list, s0, sz, nr, roff, srows, loff, l21, doff, diagT, orig_of, y, z, x, sn_prob, active);
+        wave_sync();  // the next run reuses this wavefront's LDS vectors
     }
 }
 
@@ -900,6 +960,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
         wave_forced = e && (std::string(e) == "0" || std::string(e) == "1");
         wave.assign(P.H + 1, wave_forced && std::string(e) == "1" ? 1 : 0);
     }
+    pgrid.assign(P.H + 1, kFusedGroups);
     run_q.upload(P.run_q); run_n.upload(P.run_n);
     max_sz = P.max_sz; big = P.big; fused = P.fused; G = P.G; Gc = P.Gc; Gf = P.Gf; Gb = P.Gb;
     orig_of.upload(P.orig_of); s0.upload(P.s0); sz.upload(P.sz); nr.upload(P.nr); roff.upload(P.roff); srows.upload(P.srows);
@@ -974,24 +1035,26 @@ void TriPlan::fwd_height(int h, const float* b, hipStream_t s) {
         if (fused[h]) {
             const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
             if (nr_ && wave[h]) {
+                const int grid = std::min((nr_ + 3) / 4, pgrid[h]);
                 if (run_max[h] > 1)
-                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_wave, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
                                      (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p,
                                      (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b,
                                      (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
                 else
-                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_wave, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
                                      (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p,
                                      (const float*)diag.p, (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b,
                                      (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
             } else if (nr_) {
+                const int grid = std::min(nr_, pgrid[h]);
                 if (run_max[h] > 1)
-                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
                                      (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
                                      (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
                                      (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
                 else
-                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
                                      (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
                                      (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
                                      (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
@@ -1025,24 +1088,26 @@ void TriPlan::bwd_height(int h, float* x, hipStream_t s) {
         if (fused[h]) {
             const int r0 = run_begin[h], nr_ = run_begin[h + 1] - run_begin[h];
             if (nr_ && wave[h]) {
+                const int grid = std::min((nr_ + 3) / 4, pgrid[h]);
                 if (run_max[h] > 1)
-                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_wave, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
                                      (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p,
                                      (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
                                      (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
                 else
-                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_wave, (nr_ + 3) / 4, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_wave, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p,
                                      (const int*)sn_list.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p,
                                      (const int*)srows.p, (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p,
                                      (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
             } else if (nr_) {
+                const int grid = std::min(nr_, pgrid[h]);
                 if (run_max[h] > 1)
-                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
                                      (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
                                      (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
                                      (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
                 else
-                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_small, nr_, s, r0, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
                                      (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
                                      (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
                                      (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
@@ -1166,11 +1231,22 @@ int TriPlan::tune(hipStream_t s) {
     for (int h = 0; h <= H; ++h) {
         if (fused[h]) {
             // a workgroup or a wavefront per run: the same bits, whichever is faster
-            if (wave_forced || run_begin[h + 1] == run_begin[h]) continue;
-            const float t_wg = time_height(h);
+            if (run_begin[h + 1] == run_begin[h]) continue;
+            // workgroups of the persistent launch (each loops over runs): a cap, or one per run
+            float best = time_height(h);
+            const int p0 = pgrid[h];
+            for (int cand : {1024, 2048, 4096}) {
+                const int keep = pgrid[h];
+                pgrid[h] = cand;
+                const float t = time_height(h);
+                if (t < 0.97f * best) best = t;
+                else pgrid[h] = keep;
+            }
+            changed += pgrid[h] != p0;
+            if (wave_forced) continue;
             wave[h] = 1;
             const float t_wave = time_height(h);
-            if (!(t_wave < 0.97f * t_wg)) wave[h] = 0;
+            if (!(t_wave < 0.97f * best)) wave[h] = 0;
             changed += wave[h];
             continue;
         }

@@ -59,7 +59,7 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_set_sigma_refinement", "hdd_sigma_factor_solve", "hdd_set_stream", "hdd_synchronize", "hdd_multiply",
            "hdd_inner_product", "hdd_convergence_norm", "hdd_project", "hdd_vector_copy", "hdd_vector_copy2",
            "hdd_vector_plus_equal", "hdd_vector_minus_equal", "hdd_vector_scale", "hdd_apply_preconditioner_channels",
-           "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_set_dot_order", "hdd_inner_product_noalloc"]
+           "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_set_dot_order", "hdd_inner_product_noalloc", "hdd_batches_pcg_solve"]
 
 
 def lib():
@@ -133,6 +133,7 @@ def lib():
         L.hdd_set_pcg_precision.argtypes = [P, I, I]
         L.hdd_get_x_f64.argtypes = [P, I, P]
         L.hdd_set_dot_order.argtypes = [P, I]
+        L.hdd_batches_pcg_solve.argtypes = [C.POINTER(C.c_void_p), I, C.POINTER(PcgParams), C.POINTER(PcgReport)]
         L.hdd_inner_product_noalloc.argtypes = [P, P, C.c_int64, P, P]
         for name in EXPORTS:
             if name not in ("hdd_last_error", "hdd_server_last_error"):
@@ -420,6 +421,22 @@ class DDBatch(DDSolver):
         out = np.zeros((self.problem_pages(p), 256), dtype=np.float32)
         self._check(lib().hdd_batch_get_x(self.h, p, _ptr(out)))
         return out
+
+
+def solve_concurrently(batches, tolerances, max_iterations: int = 5000):
+    """PCG for every problem of several DDBatch / DDSolver contexts at once (hdd_batches_pcg_solve):
+    each context iterates on its own stream, one iteration per context per round. tolerances: one
+    per problem, contexts in order. Returns a list of (iterations, seconds, converged) per problem."""
+    counts = [getattr(b, "count", 1) for b in batches]
+    n = sum(counts)
+    tol = np.broadcast_to(np.asarray(tolerances, dtype=np.float64), (n,))
+    prm = (PcgParams * n)(*[PcgParams(float(t), int(max_iterations)) for t in tol])
+    rep = (PcgReport * n)()
+    arr = (C.c_void_p * len(batches))(*[b.h.value for b in batches])
+    st = lib().hdd_batches_pcg_solve(arr, len(batches), prm, rep)
+    if st not in (0, 4):
+        raise HddError(f"hdd_batches_pcg_solve failed ({st}): {lib().hdd_last_error(None).decode()}")
+    return [(rep[p].iterations, rep[p].seconds, bool(rep[p].converged)) for p in range(n)]
 
 
 class Server:

@@ -108,6 +108,17 @@ hdd_status hdd_batch_page_offsets(const hdd_ctx* ctx, int problem, uint64_t* out
 hdd_status hdd_batch_pcg_solve(hdd_ctx* ctx, const hdd_pcg_params* params, hdd_pcg_report* reports,
                                double* residual_history, int history_len);
 
+/* PCG from x = 0 for every problem of several batch (or single-problem)
+ * contexts at once: each context's iterations run on its own stream, one
+ * iteration per context per round, so the contexts' kernels overlap on the
+ * device. params and reports hold one entry per problem, the contexts in
+ * order and each context's problems in order. Each problem's result is the
+ * one hdd_batch_pcg_solve gives it on its context. HDD_ERR_NOT_CONVERGED if
+ * any problem stopped at its limit; on failure hdd_last_error(NULL) and the
+ * failing context's hdd_last_error hold the message. No reference
+ * counterpart. */
+hdd_status hdd_batches_pcg_solve(hdd_ctx* const* ctxs, int count, const hdd_pcg_params* params, hdd_pcg_report* reports);
+
 /* x of one problem: n_pages(problem)*256 floats in the problem's page order. */
 hdd_status hdd_batch_get_x(hdd_ctx* ctx, int problem, float* host);
 

@@ -121,3 +121,26 @@ def test_two_copies_agree():
     print(f"[copies] iterations {res[0][0]} / {res[2][0]}, x rel diff {rel_err(x2, x0):.2e}")
     assert res[0][0] == res[2][0]
     assert rel_err(x2, x0) < 1e-6
+
+
+def test_contexts_solved_at_once_match_alone():
+    """hdd_batches_pcg_solve: two single-problem contexts and a batch iterate at once, each on its
+    own stream; every problem gives the iterations and the bits of its own context's solve."""
+    from hierarchicaldd_amd import hdd
+    from tests.gpu_common import same_bits
+    p1, p2 = domain.small_box(32, 16, 3, seed=51), domain.config0()
+    p3, p4 = domain.small_box(40, 16, 3, seed=52), domain.small_box(48, 16, 3, seed=53)
+    ctxs = [hdd.DDSolver(*p1), hdd.DDSolver(*p2), hdd.DDBatch([p3, p4])]
+    tol = [_tol(p[0]) for p in (p1, p2, p3, p4)]
+    try:
+        alone = [ctxs[0].solve(tol[0], 300)[0], ctxs[1].solve(tol[1], 300)[0]] + [r[0] for r in ctxs[2].solve_all(tol[2:], 300)]
+        xa = [ctxs[0].get(hdd.CH_X), ctxs[1].get(hdd.CH_X), ctxs[2].x(0), ctxs[2].x(1)]
+        res = hdd.solve_concurrently(ctxs, tol, 300)
+        xc = [ctxs[0].get(hdd.CH_X), ctxs[1].get(hdd.CH_X), ctxs[2].x(0), ctxs[2].x(1)]
+    finally:
+        for g in ctxs:
+            g.close()
+    print(f"[at once] iterations alone {alone}, at once {[r[0] for r in res]}", flush=True)
+    assert [r[0] for r in res] == alone
+    assert all(r[2] for r in res)
+    assert all(same_bits(a, b) for a, b in zip(xa, xc))
