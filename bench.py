@@ -407,13 +407,19 @@ def batched_leg(hdd, count, seed=0):
     with ThreadPoolExecutor(max_workers=4) as ex:
         singles = list(ex.map(lambda p: hdd.DDSolver(*p), probs))
     setup_wall = time.time() - t0
-    plain_s, setup_s, iters, xs = 0.0, 0.0, [], []
+    plain_s, setup_s, iters, xs, alone_s = 0.0, 0.0, [], [], []
     for (dom, cfg), g in zip(probs, singles):
         setup_s += g.setup_seconds
         it, hist, secs = g.solve(1e-6 * float(np.abs(dom.b).max()), 1000)
         plain_s += secs
+        alone_s.append(secs)
         iters.append(it)
         xs.append(g.get(hdd.CH_X))
+    by_edge = {}
+    for (dom, _), secs in zip(probs, alone_s):
+        by_edge.setdefault(dom.meta["n"], []).append(secs)
+    log("[batched] plain solve seconds by edge: " + ", ".join(f"{n}: {len(v)} x {np.mean(v):.3f}s = {sum(v):.2f}s"
+                                                            for n, v in sorted(by_edge.items())))
     log(f"[batched] set-up of {count} problems alone: {setup_wall:.1f}s wall, four at a time; {memory_in_use()}")
     log(f"[batched] plain: {count} solves in {plain_s:.3f}s ({count / plain_s:.2f} solves/s), host setup {setup_s:.1f}s")
     tol = [1e-6 * float(np.abs(d.b).max()) for d, _ in probs]

@@ -150,22 +150,26 @@ void partial_cholesky(double* F, int m, int ns, bool parallel) {
 }  // namespace
 
 void invert_lower(std::vector<double>& L, int n) {
+    // column c of the inverse by forward substitution, each sum in ascending k; the rows
+    // of L are read from a row-major copy so the inner loop is contiguous
+    const bool par = n > 128 && !omp_in_parallel();
+    std::vector<double> R((size_t)n * n);
+#pragma omp parallel for schedule(static) if (par)
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k <= i; ++k) R[(size_t)i * n + k] = L[(size_t)k * n + i];
     std::vector<double> X((size_t)n * n, 0.0);
     auto col = [&](int c) {
         double* x = &X[(size_t)c * n];
-        x[c] = 1.0 / L[(size_t)c * n + c];
+        x[c] = 1.0 / R[(size_t)c * n + c];
         for (int i = c + 1; i < n; ++i) {
+            const double* Ri = &R[(size_t)i * n];
             double s = 0.0;
-            for (int k = c; k < i; ++k) s += L[(size_t)k * n + i] * x[k];
-            x[i] = -s / L[(size_t)i * n + i];
+            for (int k = c; k < i; ++k) s += Ri[k] * x[k];
+            x[i] = -s / Ri[i];
         }
     };
-    if (n > 128 && !omp_in_parallel()) {
-#pragma omp parallel for schedule(dynamic, 4)
-        for (int c = 0; c < n; ++c) col(c);
-    } else {
-        for (int c = 0; c < n; ++c) col(c);
-    }
+#pragma omp parallel for schedule(dynamic, 4) if (par)
+    for (int c = 0; c < n; ++c) col(c);
     L.swap(X);
 }
 

@@ -4,8 +4,18 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 namespace hdd {
+
+// set-up phase timings on stderr when HDD_FACTOR_VERBOSE is set
+static void host_phase(const char* what, std::chrono::steady_clock::time_point& t) {
+    static const bool verbose = std::getenv("HDD_FACTOR_VERBOSE") != nullptr;
+    const auto now = std::chrono::steady_clock::now();
+    if (verbose) std::fprintf(stderr, "[setup] %s: %.3fs\n", what, std::chrono::duration<double>(now - t).count());
+    t = now;
+}
 
 DevLevel LevelBuffers::view() const {
     DevLevel v;
@@ -42,7 +52,10 @@ ProblemHost::ProblemHost(const SolverParams& p, int n_pages, const uint64_t* pag
     }
     // the interface operators are generated from the input flags, before division
     // (Initialize_Preconditioner: Split precedes Domain_Divider, DDS/...Solver.h:122-133)
+    auto tp = std::chrono::steady_clock::now();
+    host_phase("pages sorted", tp);
     generate_interface_operators(L0, P.size, P.sub, P.sigma_levels, sigma);
+    host_phase("interface operators generated", tp);
     divide_subdomains(L0, P.size, P.sub);
     set_face_flags(L0);
     for (int l = 0; l + 1 < P.vlevels; ++l) {
@@ -54,6 +67,7 @@ ProblemHost::ProblemHost(const SolverParams& p, int n_pages, const uint64_t* pag
         set_face_flags(lv[l]);
         set_boundary_flags(lv[l]);
     }
+    host_phase("divider, flags and coarse levels", tp);
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -228,7 +242,9 @@ static bool plan_local_sweeps(const PagedLevel& H, std::vector<int>& wg_page, st
 
 void Solver::build_device() {
     const int NP = nprob();
+    auto tp = std::chrono::steady_clock::now();
     for (auto& q : prob) q->factorize();
+    host_phase("factorisations", tp);
     // non-blocking: contexts of a server run side by side without the null stream serialising them
     hdd_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     upload_constants();
@@ -285,7 +301,9 @@ void Solver::build_device() {
     std::vector<const SigmaHost*> hosts;
     std::vector<const SigmaFactors*> facs;
     for (auto& q : prob) { hosts.push_back(&q->sigma); facs.push_back(q->factors.get()); }
+    host_phase("V-cycle levels uploaded", tp);
     sigma = make_sigma_device(hosts, facs, P.sig_s, P.sig_mg, stream);
+    host_phase("interface solver on the device", tp);
     hdd_check(hipStreamSynchronize(stream), "setup sync");
 }
 

@@ -23,9 +23,11 @@
 #include <mutex>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <map>
+#include <thread>
 #include <omp.h>
 #include "hdd_factor.h"
 #include "hdd_parallel.h"
@@ -34,6 +36,15 @@
 #include "hdd_spectral.h"
 
 namespace hdd {
+
+// set-up phase timings on stderr when HDD_FACTOR_VERBOSE is set
+static void setup_phase(const char* what, int level, std::chrono::steady_clock::time_point& t, const char* unit = "level") {
+    static const bool verbose = std::getenv("HDD_FACTOR_VERBOSE") != nullptr;
+    const auto now = std::chrono::steady_clock::now();
+    if (verbose) std::fprintf(stderr, "[setup] %s (%s %d): %.3fs\n", what, unit, level, std::chrono::duration<double>(now - t).count());
+    t = now;
+}
+
 
 namespace {
 
@@ -916,9 +927,18 @@ static void dense_top_inverse(const TriPlanHost::TopBlock& B, float* P) {
     const int m = B.m;
     DBuf<double> dL, dI;
     {
-        std::vector<double> L;
-        dense_top_factor(B, L);
-        dL.upload(L);
+        // filled in pinned host memory: one DMA copy of up to 1.2 GB
+        double* L = nullptr;
+        hdd_check(hipHostMalloc((void**)&L, (size_t)m * m * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        try {
+            dense_top_factor(B, L);
+            dL.alloc((size_t)m * m);
+            hdd_check(hipMemcpy(dL.p, L, (size_t)m * m * sizeof(double), hipMemcpyHostToDevice), "dense top upload");
+        } catch (...) {
+            (void)hipHostFree(L);
+            throw;
+        }
+        hdd_check(hipHostFree(L), "hipHostFree");
     }
     dI.alloc((size_t)m * m);
     static std::mutex mu;
@@ -938,7 +958,9 @@ static void dense_top_inverse(const TriPlanHost::TopBlock& B, float* P) {
 
 void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, const std::vector<int>& prob_of) {
     TriPlanHost P;
+    auto tp = std::chrono::steady_clock::now();
     build_triplan(F, offsets, P, dense_top_min_rows());
+    setup_phase("  plan lists and float blocks", (int)F.size(), tp, "factors");
     n = P.n; H = P.H; factor_nnz = P.factor_nnz;
     if (n == 0) return;
     {
@@ -970,6 +992,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
     diag.upload(P.diag); diagT.upload(P.diagT); l21.upload(P.l21);
     c.alloc(std::max<long long>(1, P.c_size)); bpart.alloc(std::max<long long>(1, P.bpart_size));
     t.alloc(n); y.alloc(2 * (size_t)n);  // forward result, then backward result
+    setup_phase("  plan uploaded", (int)F.size(), tp, "factors");
     if (!P.tops.empty()) {
         top_nq = (int)P.top_rows.size(); Gt = P.Gt;
         std::vector<long long> block_off, woff;
@@ -988,6 +1011,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
         top_t.alloc(2 * (size_t)top_nq);  // t, then y (padding stays zero)
         top_W.alloc(wsize);
         for (size_t k = 0; k < P.top_blocks.size(); ++k) dense_top_inverse(P.top_blocks[k], top_W.p + block_off[k]);
+        setup_phase("  dense tops inverted", (int)P.top_blocks.size(), tp, "blocks");
     }
 }
 
@@ -1333,37 +1357,84 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
     auto t0 = std::chrono::steady_clock::now();
     F = SigmaFactors();
     F.lv.resize(H.levels);
-    for (int l = 0; l < H.levels; ++l) {
-        SigmaFactors::Level& L = F.lv[l];
-        // subdomains of the same shape have identical Aii: factor each distinct matrix once
-        L.rep = identical_matrices(H.Aii[l]);
-        L.Aii.resize(H.nsub);
-        ParallelErrors errs;
-#pragma omp parallel for schedule(dynamic)
-        for (int s = 0; s < H.nsub; ++s)
-            if (H.Aii[l][s].rows > 0 && L.rep[s] == s) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, L.Aii[s]); });
-        errs.rethrow();
+    // The factorisations are independent: the bottom level's combined system (then its
+    // spectrum) and the level-0 interface factor each run on a thread of their own while
+    // this thread factors the interior matrices and the other interface matrices. Each
+    // factor is computed as it would be alone (the work inside one factor does not depend
+    // on what runs beside it).
+    const int lb = H.levels - 1;
+    const std::string mode = bottom_mode();
+    const bool bottom = H.levels > 0 && H.n_if[lb] > 0 && mode != "loop";
+    std::exception_ptr err_bottom, err_arr0;
+    std::thread t_bottom, t_arr0;
+    // the threads use this thread's device (dense fronts of the factorisation may run there)
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    auto same_device = [dev] { if (dev >= 0) (void)hipSetDevice(dev); };
+    auto arr_factor = [&](int l) {
+        auto tp = std::chrono::steady_clock::now();
         std::vector<Int3> rc(H.if_coords[l].size());
         for (size_t i = 0; i < rc.size(); ++i) rc[i] = {H.if_coords[l][i].x << l, H.if_coords[l][i].y << l, H.if_coords[l][i].z << l};
-        factor_spd(H.Arr[l], rc, 1e-5, L.Arr);
-        if (l == H.levels - 1 && H.n_if[l] > 0) {
-            const std::string m = bottom_mode();
-            if (m == "loop") continue;
-            Csr C; std::vector<Int3> xyz;
-            combined_bottom(H, l, C, xyz);
-            factor_spd(C, xyz, 1e-5, F.bottom);
-            F.bottom_n_int = C.rows - H.n_if[l];
-            if (m == "direct") {
-                F.bottom_direct = true;  // no low-rank correction: exact only when M^200 is negligible
-            } else {
-                bottom_spectrum(H.Arr[l], 1e-5, F.bottom, F.bottom_n_int, kBottomLambdaCut, kBottomBasisBytes, F.spec);
-                F.bottom_rho = F.spec.lambda_max; F.bottom_steps = F.spec.steps; F.bottom_seconds = F.spec.seconds;
-                F.bottom_direct = F.spec.resolved;
-                if (!F.spec.resolved) F.spec = BottomSpectrum();
+        factor_spd(H.Arr[l], rc, 1e-5, F.lv[l].Arr);
+        setup_phase("Arr factor", l, tp);
+    };
+    if (bottom)
+        t_bottom = std::thread([&] {
+            try {
+                same_device();
+                auto tp = std::chrono::steady_clock::now();
+                Csr C; std::vector<Int3> xyz;
+                combined_bottom(H, lb, C, xyz);
+                factor_spd(C, xyz, 1e-5, F.bottom);
+                F.bottom_n_int = C.rows - H.n_if[lb];
+                setup_phase("combined bottom factor", lb, tp);
+                if (mode == "direct") {
+                    F.bottom_direct = true;  // no low-rank correction: exact only when M^200 is negligible
+                } else {
+                    bottom_spectrum(H.Arr[lb], 1e-5, F.bottom, F.bottom_n_int, kBottomLambdaCut, kBottomBasisBytes, F.spec);
+                    F.bottom_rho = F.spec.lambda_max; F.bottom_steps = F.spec.steps; F.bottom_seconds = F.spec.seconds;
+                    F.bottom_direct = F.spec.resolved;
+                    if (!F.spec.resolved) F.spec = BottomSpectrum();
+                    setup_phase("bottom spectrum", lb, tp);
+                }
+                if (!F.bottom_direct) F.bottom = SupernodalFactor();
+            } catch (...) {
+                err_bottom = std::current_exception();
             }
-            if (!F.bottom_direct) F.bottom = SupernodalFactor();
+        });
+    if (H.levels > 0)
+        t_arr0 = std::thread([&] {
+            try {
+                same_device();
+                arr_factor(0);
+            } catch (...) {
+                err_arr0 = std::current_exception();
+            }
+        });
+    std::exception_ptr err_main;
+    try {
+        auto tp = t0;
+        for (int l = 0; l < H.levels; ++l) {
+            SigmaFactors::Level& L = F.lv[l];
+            // subdomains of the same shape have identical Aii: factor each distinct matrix once
+            L.rep = identical_matrices(H.Aii[l]);
+            L.Aii.resize(H.nsub);
+            ParallelErrors errs;
+#pragma omp parallel for schedule(dynamic)
+            for (int s = 0; s < H.nsub; ++s)
+                if (H.Aii[l][s].rows > 0 && L.rep[s] == s) errs.run([&] { factor_spd(H.Aii[l][s], H.Aii_coords[l][s], 1e-5, L.Aii[s]); });
+            errs.rethrow();
+            setup_phase("Aii factors", l, tp);
         }
+        for (int l = 1; l < H.levels; ++l) arr_factor(l);
+    } catch (...) {
+        err_main = std::current_exception();
     }
+    if (t_arr0.joinable()) t_arr0.join();
+    if (t_bottom.joinable()) t_bottom.join();
+    if (err_main) std::rethrow_exception(err_main);
+    if (err_arr0) std::rethrow_exception(err_arr0);
+    if (err_bottom) std::rethrow_exception(err_bottom);
     F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1411,6 +1482,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
     S->levels = NL; S->smoothing = smoothing; S->n_mg = n_mg; S->nprob = NP;
     S->lv.resize(NL);
     auto t0 = std::chrono::steady_clock::now();
+    auto tp = t0;
     // identical interior matrices across the whole batch share one factor
     std::map<uint64_t, std::vector<std::pair<const Csr*, const SupernodalFactor*>>> canon;
     auto canonical = [&](const Csr& A, const SupernodalFactor* f) {
@@ -1455,9 +1527,13 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         L.AirT.upload(transpose(air));
         L.Arr.upload(arr);
         if (l + 1 < NL) { pr.cols = ifoff[l + 1][NP]; L.P.upload(pr); L.PT.upload(transpose(pr)); }
+        setup_phase("interface operators uploaded", l, tp);
         L.Lii.build(fi, oi, pi);
+        setup_phase("Aii plan", l, tp);
         L.Lrr.build(fr, orr, prr);
+        setup_phase("Arr plan", l, tp);
         if (autotune()) S->tuned_choices += L.Lii.tune(stream) + L.Lrr.tune(stream);
+        setup_phase("plans tuned", l, tp);
         S->dense_top_rows += L.Lii.top_dense_rows + L.Lrr.top_dense_rows;
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
@@ -1503,8 +1579,11 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
             S->spec_nch = std::max(S->spec_nch, (np + kSpecChunk - 1) / kSpecChunk);
             boff += F.bottom_n_int + np;
         }
+        setup_phase("bottom modes gathered", lb, tp);
         S->Lbottom.build(fb, ob, pb);
+        setup_phase("bottom plan", lb, tp);
         if (autotune()) S->tuned_choices += S->Lbottom.tune(stream);
+        setup_phase("bottom plan tuned", lb, tp);
         S->dense_top_rows += S->Lbottom.top_dense_rows;
         S->bot_rhs.alloc(boff); S->bot_sol.alloc(boff);
         S->bot_pos.upload(pos);
@@ -1523,6 +1602,7 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
     }
     if (S->bottom_direct) {
         S->bottom_check = bottom_self_check(*S, stream);
+        setup_phase("bottom self-check", lb, tp);
         // the reference's sweeps stay in charge if the shortcut disagrees with them
         if (bottom_mode() != "direct" && !(S->bottom_check < 1e-3)) S->bottom_direct = false;
     }

@@ -50,7 +50,7 @@ int dense_top_cut(const SupernodalFactor& f, int top_min_rows, int members, int*
     return cut;
 }
 
-void dense_top_factor(const TriPlanHost::TopBlock& B, std::vector<double>& L) {
+void dense_top_factor(const TriPlanHost::TopBlock& B, double* L) {
     const SupernodalFactor& f = *B.f;
     const int m = B.m;
     // S-local index of each permuted row of S: rows of S ascending
@@ -63,21 +63,33 @@ void dense_top_factor(const TriPlanHost::TopBlock& B, std::vector<double>& L) {
         for (int i = 0; i < f.n; ++i) if (in[i]) loc[i] = q++;
         if (q != m) throw std::runtime_error("dense top: row count mismatch");
     }
-    L.assign((size_t)m * m, 0.0);
-    for (size_t s = 0; s < f.sn.size(); ++s) {
-        const Supernode& S = f.sn[s];
-        if (S.height < B.cut) continue;
-        const int n = S.n, nr = (int)S.rows.size();
-        for (int j = 0; j < n; ++j) {
-            double* col = &L[(size_t)loc[S.s0 + j] * m];
-            for (int r = j; r < n; ++r) col[loc[S.s0 + r]] = f.L11[s][(size_t)j * n + r];
-            for (int a = 0; a < nr; ++a) {
-                const int li = loc[S.rows[a]];
-                if (li < 0) throw std::runtime_error("dense top: a row below the top is not in it");
-                col[li] = f.L21[s][(size_t)j * nr + a];
+    bool outside = false;  // a sub-diagonal row of a top supernode below the top (the cut is not ancestor-closed)
+#pragma omp parallel
+    {
+#pragma omp for schedule(static)
+        for (int j = 0; j < m; ++j) std::fill(L + (size_t)j * m, L + (size_t)(j + 1) * m, 0.0);
+        // each column belongs to one supernode: the supernodes fill disjoint columns
+#pragma omp for schedule(dynamic)
+        for (size_t s = 0; s < f.sn.size(); ++s) {
+            const Supernode& S = f.sn[s];
+            if (S.height < B.cut) continue;
+            const int n = S.n, nr = (int)S.rows.size();
+            for (int j = 0; j < n; ++j) {
+                double* col = L + (size_t)loc[S.s0 + j] * m;
+                for (int r = j; r < n; ++r) col[loc[S.s0 + r]] = f.L11[s][(size_t)j * n + r];
+                for (int a = 0; a < nr; ++a) {
+                    const int li = loc[S.rows[a]];
+                    if (li < 0) {
+#pragma omp atomic write
+                        outside = true;
+                        continue;
+                    }
+                    col[li] = f.L21[s][(size_t)j * nr + a];
+                }
             }
         }
     }
+    if (outside) throw std::runtime_error("dense top: a row below the top is not in it");
 }
 
 void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vector<int>& offsets, TriPlanHost& P, int top_min_rows) {
@@ -156,9 +168,9 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     }
     const int nsn = P.nsn = (int)P.s0.size();
     P.diag.assign(dsize, 0.f); P.diagT.assign(dsize, 0.f); P.l21.assign(lsize, 0.f);
-#pragma omp parallel for schedule(dynamic)
-    for (int g = 0; g < nsn; ++g) {
-        if (!owner[g]) continue;
+    // diagonal-block inverses and the sub-diagonal blocks in float: small supernodes one per
+    // thread, then the big ones one at a time with the threads over their columns
+    auto fill_blocks = [&](int g) {
         const SupernodalFactor& f = *F[refs[g].m];
         const int s = refs[g].s, ns = P.sz[g], nr = P.nr[g];
         std::vector<double> inv(f.L11[s]);
@@ -174,7 +186,13 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         const std::vector<double>& src = f.L21[s];  // column-major nr x ns
         for (int a = 0; a < nr; ++a)
             for (int j = 0; j < ns; ++j) L[(size_t)a * ns + j] = (float)src[(size_t)j * nr + a];
-    }
+    };
+    constexpr int kBigInverse = 256;
+#pragma omp parallel for schedule(dynamic)
+    for (int g = 0; g < nsn; ++g)
+        if (owner[g] && P.sz[g] <= kBigInverse) fill_blocks(g);
+    for (int g = 0; g < nsn; ++g)
+        if (owner[g] && P.sz[g] > kBigInverse) fill_blocks(g);
     // contributions and tile partials per supernode
     P.cofs.assign(nsn, 0); P.bofs.assign(nsn, 0);
     for (int g = 0; g < nsn; ++g) {

@@ -114,8 +114,9 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
 int dense_top_cut(const SupernodalFactor& f, int top_min_rows, int members, int* m);
 
 // Dense L_SS of a top block, m x m column-major in double (rows of S in
-// ascending permuted order; S is ancestor-closed, so L_SS is lower triangular).
-void dense_top_factor(const TriPlanHost::TopBlock& B, std::vector<double>& L);
+// ascending permuted order; S is ancestor-closed, so L_SS is lower triangular),
+// written to L (m * m doubles, every entry set).
+void dense_top_factor(const TriPlanHost::TopBlock& B, double* L);
 
 // Rows a factor needs to have a dense top by default, or HDD_DENSE_TOP from the
 // environment (0 turns dense tops off).
