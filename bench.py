@@ -20,8 +20,8 @@ solve), set beside the GPU's rate on the same 64^3 problem.
 
 The other two parts of the metric are added to the same line:
   "batched": uneven independent problems (edge 64-256, 1-4 sphere obstacles,
-      delta right-hand side, sub_size 32; --batch-count of them, 16 by default,
-      64 for the full set) solved one at a time on the GPU, then all at once in
+      delta right-hand side, sub_size 32; --batch-count of them, 64 by default)
+      solved one at a time on the GPU, then all at once in
       three forms: every problem's own context iterating at once on its own
       stream, one batched context (every stage one launch for all problems),
       and four batched contexts of similar-size problems iterating at once;
@@ -86,11 +86,12 @@ def main():
                     help="sparse512: the 512^3 union-of-spheres domain, FP32 and FP16-storage V-cycles at 1e-6 and 1e-8")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--max-iterations", type=int, default=500)
-    ap.add_argument("--batch-count", type=int, default=16, help="problems of the batched leg (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=8, help="requests per rate of the served leg (0: skip)")
-    ap.add_argument("--serve-rates", default="0.5,2,8", help="comma-separated Poisson arrival rates (requests/s) of the served leg")
+    ap.add_argument("--batch-count", type=int, default=64, help="problems of the batched leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=32, help="requests per rate of the served leg (0: skip)")
+    ap.add_argument("--serve-rates", default="8", help="comma-separated Poisson arrival rates (requests/s) of the served leg")
     ap.add_argument("--serve-nmax", type=int, default=256,
-                    help="largest request edge of the served leg (384 for the full stream; 256 keeps the default run short)")
+                    help="largest request edge of the served leg (384 for the full stream: its host set-up alone takes "
+                         "about 6 minutes per pass over 32 requests, so the default run stops at 256)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="set up, then run only level-0 sweeps and a plain copy of known size (for counter runs)")
     ap.add_argument("--plan-only", default=None, metavar="LEVEL,PLAN",

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 
 namespace hdd {
 
@@ -54,30 +55,55 @@ ProblemHost::ProblemHost(const SolverParams& p, int n_pages, const uint64_t* pag
     // (Initialize_Preconditioner: Split precedes Domain_Divider, DDS/...Solver.h:122-133)
     auto tp = std::chrono::steady_clock::now();
     host_phase("pages sorted", tp);
-    generate_interface_operators(L0, P.size, P.sub, P.sigma_levels, sigma);
-    host_phase("interface operators generated", tp);
-    divide_subdomains(L0, P.size, P.sub);
-    set_face_flags(L0);
-    for (int l = 0; l + 1 < P.vlevels; ++l) {
-        const Int3 s = lv[l].size;
-        lv[l + 1].init({s.x / 2 + 1, s.y / 2 + 1, s.z / 2 + 1});
-        downsample_level(lv[l], lv[l + 1]);
+    // The interface operators are generated from the input flags, the divider and the
+    // V-cycle levels rewrite them: the generator reads a copy on this thread while the
+    // levels are built on another (neither reads what the other writes).
+    {
+        const PagedLevel input = L0;
+        std::exception_ptr err;
+        std::thread levels([&] {
+            try {
+                auto tl = std::chrono::steady_clock::now();
+                divide_subdomains(L0, P.size, P.sub);
+                set_face_flags(L0);
+                for (int l = 0; l + 1 < P.vlevels; ++l) {
+                    const Int3 s = lv[l].size;
+                    lv[l + 1].init({s.x / 2 + 1, s.y / 2 + 1, s.z / 2 + 1});
+                    downsample_level(lv[l], lv[l + 1]);
+                }
+                for (int l = 0; l < P.vlevels; ++l) {
+                    set_face_flags(lv[l]);
+                    set_boundary_flags(lv[l]);
+                }
+                host_phase("divider, flags and coarse levels", tl);
+            } catch (...) {
+                err = std::current_exception();
+            }
+        });
+        try {
+            generate_interface_operators(input, P.size, P.sub, P.sigma_levels, sigma);
+        } catch (...) {
+            levels.join();
+            throw;
+        }
+        host_phase("interface operators generated", tp);
+        levels.join();
+        if (err) std::rethrow_exception(err);
     }
-    for (int l = 0; l < P.vlevels; ++l) {
-        set_face_flags(lv[l]);
-        set_boundary_flags(lv[l]);
-    }
-    host_phase("divider, flags and coarse levels", tp);
+    host_phase("both done", tp);
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 ProblemHost::~ProblemHost() {}
 
-void ProblemHost::factorize() {
-    if (factors) return;
+void ProblemHost::factorize(bool wait) {
+    if (factors) {
+        if (wait) factors->wait_bottom();
+        return;
+    }
     auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<SigmaFactors> f(new SigmaFactors());
-    factor_sigma(sigma, *f);
+    factor_sigma(sigma, *f, wait);
     factors = std::move(f);
     factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -243,8 +269,9 @@ static bool plan_local_sweeps(const PagedLevel& H, std::vector<int>& wg_page, st
 void Solver::build_device() {
     const int NP = nprob();
     auto tp = std::chrono::steady_clock::now();
-    for (auto& q : prob) q->factorize();
-    host_phase("factorisations", tp);
+    // the bottom levels' factors and spectra finish while the levels and plans above them are built
+    for (auto& q : prob) q->factorize(false);
+    host_phase("factorisations (bottom levels still running)", tp);
     // non-blocking: contexts of a server run side by side without the null stream serialising them
     hdd_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     upload_constants();

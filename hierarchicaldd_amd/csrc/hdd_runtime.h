@@ -84,7 +84,9 @@ struct ProblemHost {
 
     ProblemHost(const SolverParams& p, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, const float* b);
     ~ProblemHost();
-    void factorize();                  // idempotent
+    // idempotent; with wait = false the bottom level's factor may still be in progress on
+    // return (factors->wait_bottom() before reading it)
+    void factorize(bool wait = true);
     // device bytes of the problem inside a batch (pages of every level, vectors, interface plans)
     size_t device_bytes() const;
 };

@@ -1,6 +1,8 @@
 // Device-side sigma (interface Schur complement) multigrid.
 #pragma once
+#include <exception>
 #include <memory>
+#include <thread>
 #include <vector>
 #include "hdd_device.h"
 #include "hdd_factor.h"
@@ -94,8 +96,19 @@ struct SigmaFactors {
     int bottom_steps = 0;
     double bottom_seconds = 0;
     double seconds = 0;
+    // factor_sigma(..., wait = false) leaves the bottom level's factor and spectrum to a
+    // worker thread: wait_bottom() joins it (rethrowing its error) before they are read.
+    // Joining is not thread-safe: one thread waits (the one that builds the device plans).
+    void wait_bottom() const;
+    SigmaFactors() = default;
+    SigmaFactors(const SigmaFactors&) = delete;
+    SigmaFactors& operator=(const SigmaFactors&) = delete;
+    ~SigmaFactors();
+    mutable std::thread bottom_worker;
+    mutable std::exception_ptr bottom_error;
 };
-void factor_sigma(const SigmaHost& H, SigmaFactors& F);
+// F must be freshly constructed; the worker (wait = false) reads H until it is joined
+void factor_sigma(const SigmaHost& H, SigmaFactors& F, bool wait = true);
 
 struct SigmaDevice {
     int levels = 0, smoothing = 3, n_mg = 1, nprob = 1;

@@ -1353,10 +1353,18 @@ static std::string bottom_mode() {
     return m;
 }
 
-void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
+void SigmaFactors::wait_bottom() const {
+    if (bottom_worker.joinable()) bottom_worker.join();
+    if (bottom_error) std::rethrow_exception(bottom_error);
+}
+
+SigmaFactors::~SigmaFactors() {
+    if (bottom_worker.joinable()) bottom_worker.join();
+}
+
+void factor_sigma(const SigmaHost& H, SigmaFactors& F, bool wait) {
     auto t0 = std::chrono::steady_clock::now();
-    F = SigmaFactors();
-    F.lv.resize(H.levels);
+    F.lv.assign(H.levels, SigmaFactors::Level());
     // The factorisations are independent: the bottom level's combined system (then its
     // spectrum) and the level-0 interface factor each run on a thread of their own while
     // this thread factors the interior matrices and the other interface matrices. Each
@@ -1365,8 +1373,8 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
     const int lb = H.levels - 1;
     const std::string mode = bottom_mode();
     const bool bottom = H.levels > 0 && H.n_if[lb] > 0 && mode != "loop";
-    std::exception_ptr err_bottom, err_arr0;
-    std::thread t_bottom, t_arr0;
+    std::exception_ptr err_arr0;
+    std::thread t_arr0;
     // the threads use this thread's device (dense fronts of the factorisation may run there)
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) dev = -1;
@@ -1379,7 +1387,7 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
         setup_phase("Arr factor", l, tp);
     };
     if (bottom)
-        t_bottom = std::thread([&] {
+        F.bottom_worker = std::thread([&H, &F, lb, mode, same_device] {
             try {
                 same_device();
                 auto tp = std::chrono::steady_clock::now();
@@ -1399,7 +1407,7 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
                 }
                 if (!F.bottom_direct) F.bottom = SupernodalFactor();
             } catch (...) {
-                err_bottom = std::current_exception();
+                F.bottom_error = std::current_exception();
             }
         });
     if (H.levels > 0)
@@ -1431,10 +1439,9 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F) {
         err_main = std::current_exception();
     }
     if (t_arr0.joinable()) t_arr0.join();
-    if (t_bottom.joinable()) t_bottom.join();
+    if (wait || err_main || err_arr0) F.wait_bottom();
     if (err_main) std::rethrow_exception(err_main);
     if (err_arr0) std::rethrow_exception(err_arr0);
-    if (err_bottom) std::rethrow_exception(err_bottom);
     F.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1532,14 +1539,22 @@ std::unique_ptr<SigmaDevice> make_sigma_device(const std::vector<const SigmaHost
         setup_phase("Aii plan", l, tp);
         L.Lrr.build(fr, orr, prr);
         setup_phase("Arr plan", l, tp);
-        if (autotune()) S->tuned_choices += L.Lii.tune(stream) + L.Lrr.tune(stream);
-        setup_phase("plans tuned", l, tp);
         S->dense_top_rows += L.Lii.top_dense_rows + L.Lrr.top_dense_rows;
         L.u.alloc(std::max(1, L.n_if)); L.b.alloc(std::max(1, L.n_if)); L.r.alloc(std::max(1, L.n_if)); L.tmp.alloc(std::max(1, L.n_if));
         L.vb.alloc(std::max(1, L.n_int)); L.vx.alloc(std::max(1, L.n_int));
-        S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes() + L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
+        S->plan_bytes += L.Air.bytes() + L.AirT.bytes() + L.Arr.bytes() + L.P.bytes() + L.PT.bytes();
     }
     // bottom level: one block of the combined plan and one set of modes per problem
+    // bottom factors and spectra still being computed (factor_sigma without wait) are awaited here
+    for (const SigmaFactors* f : factors) f->wait_bottom();
+    setup_phase("bottom factors and spectra awaited", NL - 1, tp);
+    // launch shapes are timed once no set-up work (dense fronts of the bottom factors) shares the GPU
+    for (int l = 0; l < NL; ++l) {
+        SigmaLevel& L = S->lv[l];
+        if (autotune()) S->tuned_choices += L.Lii.tune(stream) + L.Lrr.tune(stream);
+        S->plan_bytes += L.Lii.bytes() + L.Lrr.bytes();  // with the tuner's list capacity
+        setup_phase("plans tuned", l, tp);
+    }
     const int lb = NL - 1;
     bool any_if = false, all_direct = true;
     for (int p = 0; p < NP; ++p) {

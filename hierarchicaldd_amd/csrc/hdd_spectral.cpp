@@ -4,6 +4,8 @@
 #include <cmath>
 #include <limits>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <omp.h>
 
@@ -638,9 +640,26 @@ void lanczos_spectrum(const Csr& Arr, double eps, const SupernodalFactor& Fc, in
     int nb = 1, prev_above = -1;
     std::vector<double> Y, theta, res;
     int next_check = 4;
+    // time spent in the block solves, the orthogonalisation and the Rayleigh-Ritz steps
+    // (on stderr with HDD_FACTOR_VERBOSE)
+    double t_solve = 0, t_orth = 0, t_ritz = 0;
+    auto tick = std::chrono::steady_clock::now();
+    auto lap = [&tick](double& acc) {
+        const auto now = std::chrono::steady_clock::now();
+        acc += std::chrono::duration<double>(now - tick).count();
+        tick = now;
+    };
+    struct Report {
+        const double &s, &o, &r;
+        ~Report() {
+            if (std::getenv("HDD_FACTOR_VERBOSE")) std::fprintf(stderr, "[setup]   bottom spectrum: block solves %.3fs, orthogonalisation %.3fs, Rayleigh-Ritz %.3fs\n", s, o, r);
+        }
+    } report{t_solve, t_orth, t_ritz};
     while (nb * p + p <= ld) {
         const int j = nb - 1, cols = nb * p;
+        lap(t_orth);
         solve_schur(Fs, Fc, n_int, n, p, &SQ[(size_t)j * p * n], W.data());  // W = T Q_j
+        lap(t_solve);
         for (int pass = 0; pass < 2; ++pass) {
             gemm_tn(n, cols, p, SQ.data(), W.data(), C.data());
             gemm_nn_sub(n, cols, p, Q.data(), C.data(), W.data());
@@ -657,6 +676,7 @@ void lanczos_spectrum(const Csr& Arr, double eps, const SupernodalFactor& Fc, in
         const int m = nb - 1;  // blocks in the Rayleigh-Ritz space; block m is the residual direction
         const bool last = nb * p + p > ld;
         if (m < next_check && !last) continue;
+        lap(t_orth);
         next_check = m + std::max(2, m / 4);
         const int mc = m * p;
         Y.assign((size_t)mc * mc, 0.0);
@@ -698,6 +718,7 @@ void lanczos_spectrum(const Csr& Arr, double eps, const SupernodalFactor& Fc, in
             return;
         }
         prev_above = all_conv ? above : -1;
+        lap(t_ritz);
         if (last) break;
     }
     out.resolved = false;
