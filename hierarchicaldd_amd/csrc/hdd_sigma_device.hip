@@ -243,8 +243,10 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
 // factors; runs of one otherwise) does the whole height's work for them in one
 // launch. Every block entry is loaded once and applied to each member in turn,
 // and each member's sums run in exactly the order of a one-member run.
-// forward: t = b - (incidence sums) -> y = Linv t (LDS) -> c_g = L21_g y
-template <int G, int M>
+// forward: t = b - (incidence sums) -> y = Linv t (LDS) -> c_g = L21_g y.
+// With WF the block passed as l21 is W_g = L21_g Linv_g (formed in double at
+// set-up), so c_g = W_g t needs only t: both products run in one phase.
+template <int G, int M, bool WF>
 __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -308,7 +310,7 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
                 ys[r][j] = 0.f;
             }
     }
-    __syncthreads();
+    if (!WF) __syncthreads();  // WF: the contributions read t, complete since the gather
     const float* L = l21 + loff[g0];
     for (int a = (int)threadIdx.x / G; a < m; a += 256 / G) {
         const float* La = L + (long long)a * n;
@@ -322,7 +324,7 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
                 const float l = La[k];
                 float yk[M];
 #pragma unroll
-                for (int j = 0; j < M; ++j) yk[j] = ys[k][j];
+                for (int j = 0; j < M; ++j) yk[j] = WF ? ts[k][j] : ys[k][j];
 #pragma unroll
                 for (int j = 0; j < M; ++j)
                     if (j < cnt) acc[j] += l * yk[j];
@@ -337,7 +339,7 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
     }
 }
 
-template <int G, int M>
+template <int G, int M, bool WF>
 __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -348,15 +350,17 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, int nruns, const 
                                                        const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     for (int run = blockIdx.x; run < nruns; run += gridDim.x) {
-        fwd_small_run<G, M>(run, r0, run_q, run_n, sn_list, s0, sz, nr, doff, diag, inc_ptr, cpos, orig_of, b, loff, cofs, l21, c, y, sn_prob, active);
+        fwd_small_run<G, M, WF>(run, r0, run_q, run_n, sn_list, s0, sz, nr, doff, diag, inc_ptr, cpos, orig_of, b, loff, cofs, l21, c, y, sn_prob, active);
         __syncthreads();  // the next run reuses the LDS vectors
     }
 }
 
 // backward: t_j = y_j - sum_a L21_g[a, j] z[rows_g[a]] (four wavefronts split
 // the rows: wavefront w takes rows a = w mod 4 in ascending order; lanes own
-// columns j and j + 64; LDS reduction) -> z = Linv^T t
-template <int G, int M>
+// columns j and j + 64; LDS reduction) -> z = Linv^T t. With WF the block is
+// W_g = L21_g Linv_g and z = Linv^T y - W^T z[rows]: the diagonal product needs
+// no ancestor value and runs beside the tile sums.
+template <int G, int M, bool WF>
 __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -370,6 +374,7 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
     __shared__ __align__(16) float zr[kChunk][M];  // member-minor, as ts
     __shared__ float part[4][M][TriPlanHost::kSliceRows];
     __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
+    __shared__ __align__(16) float z1[WF ? TriPlanHost::kSliceRows : 1][M];  // WF: Linv^T y
     __shared__ int base_s[M], roff_s[M];
     const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
@@ -384,6 +389,48 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
 #pragma unroll
     for (int j = 0; j < M; ++j) { acc0[j] = 0.f; acc1[j] = 0.f; }
     __syncthreads();
+    const float* DT = diagT + doff[g0];
+    const int gl = threadIdx.x & (G - 1);
+    constexpr int kU = TriPlanHost::kSliceRows / G;
+    // WF: Linv^T y into z1, rows and sums as in the diagonal step below
+    auto diag_of_y = [&]() {
+        for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
+            const float* Dr = DT + (long long)r * n;
+            float acc[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) acc[j] = 0.f;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int k = r + gl + u * G;
+                if (k < n) {
+                    const float d = Dr[k];
+                    float tk[M];
+#pragma unroll
+                    for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
+#pragma unroll
+                    for (int j = 0; j < M; ++j)
+                        if (j < cnt) acc[j] += d * tk[j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < cnt) {
+                    const float v = group_sum<G>(acc[j]);
+                    if (gl == 0) z1[r][j] = v;
+                }
+        }
+    };
+    if (WF) {
+        // t = y, visible after the first chunk's barrier (or the one below without sub-diagonal rows)
+        for (int idx = threadIdx.x; idx < M * n; idx += 256) {
+            const int j = idx / n, k = idx - j * n;
+            ts[k][j] = j < cnt ? y[base_s[j] + k] : 0.f;
+        }
+        if (m == 0) {
+            __syncthreads();
+            diag_of_y();
+        }
+    }
     for (int a0 = 0; a0 < m; a0 += kChunk) {
         const int cm = min(kChunk, m - a0);
         for (int idx = threadIdx.x; idx < M * cm; idx += 256) {
@@ -391,6 +438,7 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
             zr[q][j] = j < cnt ? z[srows[roff_s[j] + a0 + q]] : 0.f;
         }
         __syncthreads();
+        if (WF && a0 == 0) diag_of_y();
 #pragma unroll 4
         for (int q = w; q < cm; q += 4) {
             const float* La = L + (long long)(a0 + q) * n;
@@ -414,14 +462,22 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
             if (lane + 64 < n) part[w][j][lane + 64] = acc1[j];
         }
     __syncthreads();
+    if (WF) {
+        for (int idx = threadIdx.x; idx < M * n; idx += 256) {
+            const int j = idx / n, k = idx - j * n;
+            if (j < cnt) {
+                const float v = z1[k][j] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k]));
+                z[base_s[j] + k] = v;
+                x[orig_of[base_s[j] + k]] = v;
+            }
+        }
+        return;
+    }
     for (int idx = threadIdx.x; idx < M * n; idx += 256) {
         const int j = idx / n, k = idx - j * n;
         ts[k][j] = j < cnt ? y[base_s[j] + k] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k])) : 0.f;
     }
     __syncthreads();
-    const float* DT = diagT + doff[g0];
-    const int gl = threadIdx.x & (G - 1);
-    constexpr int kU = TriPlanHost::kSliceRows / G;
     for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
         const float* Dr = DT + (long long)r * n;
         float acc[M];
@@ -449,7 +505,7 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
     }
 }
 
-template <int G, int M>
+template <int G, int M, bool WF>
 __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -460,7 +516,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, int nruns, const 
                                                        float* __restrict__ z, float* __restrict__ x,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     for (int run = blockIdx.x; run < nruns; run += gridDim.x) {
-        bwd_small_run<G, M>(run, r0, run_q, run_n, sn_list, s0, sz, nr, roff, srows, loff, l21, doff, diagT, orig_of, y, z, x, sn_prob, active);
+        bwd_small_run<G, M, WF>(run, r0, run_q, run_n, sn_list, s0, sz, nr, roff, srows, loff, l21, doff, diagT, orig_of, y, z, x, sn_prob, active);
         __syncthreads();  // the next run reuses the LDS vectors
     }
 }
@@ -981,6 +1037,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
         const char* e = std::getenv("HDD_FUSED_WAVE");
         wave_forced = e && (std::string(e) == "0" || std::string(e) == "1");
         wave.assign(P.H + 1, wave_forced && std::string(e) == "1" ? 1 : 0);
+        wform = !wave_forced && !P.w21.empty();
     }
     pgrid.assign(P.H + 1, kFusedGroups);
     run_q.upload(P.run_q); run_n.upload(P.run_n);
@@ -990,6 +1047,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
     ci_sn.upload(P.ci_sn); ci_a0.upload(P.ci_a0); ci_a1.upload(P.ci_a1); bi_sn.upload(P.bi_sn); bi_c.upload(P.bi_c); bi_j0.upload(P.bi_j0);
     loff.upload(P.loff); doff.upload(P.doff); cofs.upload(P.cofs); cpos.upload(P.cpos); bofs.upload(P.bofs);
     diag.upload(P.diag); diagT.upload(P.diagT); l21.upload(P.l21);
+    if (wform) { woff.upload(P.woff); w21.upload(P.w21); }
     c.alloc(std::max<long long>(1, P.c_size)); bpart.alloc(std::max<long long>(1, P.bpart_size));
     t.alloc(n); y.alloc(2 * (size_t)n);  // forward result, then backward result
     setup_phase("  plan uploaded", (int)F.size(), tp, "factors");
@@ -1018,7 +1076,7 @@ void TriPlan::build(const std::vector<const SupernodalFactor*>& F, const std::ve
 size_t TriPlan::bytes() const {
     return (run_q.n + run_n.n + orig_of.n + s0.n + sz.n + nr.n + roff.n + srows.n + sn_of.n + sn_prob.n + sn_list.n + sl_sn.n + sl_r0.n + rowlist.n + inc_ptr.n + ci_sn.n + ci_a0.n +
             ci_a1.n + bi_sn.n + bi_c.n + bi_j0.n) * 4 +
-           (loff.n + doff.n + cofs.n + cpos.n + bofs.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n) * 4 +
+           (loff.n + doff.n + cofs.n + cpos.n + bofs.n + woff.n) * 8 + (diag.n + diagT.n + l21.n + c.n + bpart.n + t.n + y.n + w21.n) * 4 +
            (top_rows.n + top_of.n + top_inc_ptr.n + top_base.n + top_ld.n + top_W.n + top_t.n) * 4 + (top_cpos.n + top_woff.n) * 8;
 }
 
@@ -1041,6 +1099,28 @@ static constexpr int kRunM = TriPlanHost::kRunMembers;
             case 32: hipLaunchKernelGGL((KERNEL<32, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
             default: hipLaunchKernelGGL((KERNEL<64, M>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
         }                                                                                                       \
+    } while (0)
+
+#define HDD_RUN_WF_DISPATCH(G, M, WF, KERNEL, BLOCKS, STREAM, ...)                                                  \
+    do {                                                                                                            \
+        switch (G) {                                                                                                \
+            case 4: hipLaunchKernelGGL((KERNEL<4, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break;   \
+            case 8: hipLaunchKernelGGL((KERNEL<8, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break;   \
+            case 16: hipLaunchKernelGGL((KERNEL<16, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            case 32: hipLaunchKernelGGL((KERNEL<32, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<64, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+        }                                                                                                           \
+    } while (0)
+// the fused workgroup kernels: run length (runs of one or of up to kRunM) x block form (L21 or W)
+#define HDD_FUSED_DISPATCH(H, WF, KERNEL, BLOCKS, STREAM, ...)                                                       \
+    do {                                                                                                            \
+        if (run_max[H] > 1) {                                                                                       \
+            if (WF) HDD_RUN_WF_DISPATCH(G[H], kRunM, true, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                    \
+            else HDD_RUN_WF_DISPATCH(G[H], kRunM, false, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                      \
+        } else {                                                                                                    \
+            if (WF) HDD_RUN_WF_DISPATCH(G[H], 1, true, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                        \
+            else HDD_RUN_WF_DISPATCH(G[H], 1, false, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                          \
+        }                                                                                                           \
     } while (0)
 
 #define HDD_ROWS_DISPATCH(G, KERNEL, ROWS, ...)                                                                \
@@ -1072,16 +1152,13 @@ void TriPlan::fwd_height(int h, const float* b, hipStream_t s) {
                                      (const long long*)loff.p, (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
             } else if (nr_) {
                 const int grid = std::min(nr_, pgrid[h]);
-                if (run_max[h] > 1)
-                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_fwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
-                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
-                                     (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
-                                     (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
-                else
-                    HDD_RUN_DISPATCH(G[h], 1, k_tri_fwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
-                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
-                                     (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, (const long long*)loff.p,
-                                     (const long long*)cofs.p, (const float*)l21.p, c.p, yf, SP, active);
+                const bool wf = use_w();
+                const long long* blo = wf ? woff.p : loff.p;
+                const float* blk = wf ? w21.p : l21.p;
+                HDD_FUSED_DISPATCH(h, wf, k_tri_fwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                                   (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p, (const float*)diag.p,
+                                   (const int*)inc_ptr.p, (const long long*)cpos.p, (const int*)orig_of.p, b, blo, (const long long*)cofs.p,
+                                   blk, c.p, yf, SP, active);
             }
             return;
         }
@@ -1125,16 +1202,12 @@ void TriPlan::bwd_height(int h, float* x, hipStream_t s) {
                                      (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
             } else if (nr_) {
                 const int grid = std::min(nr_, pgrid[h]);
-                if (run_max[h] > 1)
-                    HDD_RUN_DISPATCH(G[h], kRunM, k_tri_bwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
-                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
-                                     (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
-                                     (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
-                else
-                    HDD_RUN_DISPATCH(G[h], 1, k_tri_bwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
-                                     (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p,
-                                     (const long long*)loff.p, (const float*)l21.p, (const long long*)doff.p, (const float*)diagT.p,
-                                     (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
+                const bool wf = use_w();
+                const long long* blo = wf ? woff.p : loff.p;
+                const float* blk = wf ? w21.p : l21.p;
+                HDD_FUSED_DISPATCH(h, wf, k_tri_bwd_small, grid, s, r0, nr_, (const int*)run_q.p, (const int*)run_n.p, (const int*)sn_list.p,
+                                   (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const int*)roff.p, (const int*)srows.p, blo, blk,
+                                   (const long long*)doff.p, (const float*)diagT.p, (const int*)orig_of.p, (const float*)yf, zb, x, SP, active);
             }
             return;
         }
@@ -1254,7 +1327,7 @@ int TriPlan::tune(hipStream_t s) {
     int changed = 0;
     for (int h = 0; h <= H; ++h) {
         if (fused[h]) {
-            // a workgroup or a wavefront per run: the same bits, whichever is faster
+            // a workgroup or a wavefront per run (L21 form): the same bits, whichever is faster
             if (run_begin[h + 1] == run_begin[h]) continue;
             // workgroups of the persistent launch (each loops over runs): a cap, or one per run
             float best = time_height(h);
@@ -1267,7 +1340,8 @@ int TriPlan::tune(hipStream_t s) {
                 else pgrid[h] = keep;
             }
             changed += pgrid[h] != p0;
-            if (wave_forced) continue;
+            // the wavefront form has the L21 blocks only: not a candidate beside the W form
+            if (wave_forced || wform) continue;
             wave[h] = 1;
             const float t_wave = time_height(h);
             if (!(t_wave < 0.97f * best)) wave[h] = 0;

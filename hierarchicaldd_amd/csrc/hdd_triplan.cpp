@@ -167,6 +167,23 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         }
     }
     const int nsn = P.nsn = (int)P.s0.size();
+    // heights done by the fused kernels (the same rule as P.fused below): their supernodes get W
+    std::vector<char> fused_h(P.H + 1, 0);
+    {
+        std::vector<int> count(P.H + 1, 0), mx(P.H + 1, 0);
+        for (int g = 0; g < nsn; ++g)
+            if (!in_top[g]) { ++count[height[g]]; mx[height[g]] = std::max(mx[height[g]], P.sz[g]); }
+        for (int h = 0; h <= P.H; ++h)
+            fused_h[h] = mx[h] <= TriPlanHost::kSliceRows && count[h] >= TriPlanHost::kFusedMinSupernodes;
+    }
+    P.woff.assign(nsn, 0);
+    long long wsize = 0;
+    for (int g = 0; g < nsn; ++g) {
+        if (in_top[g] || !fused_h[height[g]]) continue;
+        if (owner[g]) { P.woff[g] = wsize; wsize += (long long)P.sz[g] * P.nr[g]; }
+        else P.woff[g] = P.woff[first_gid.at(F[refs[g].m]) + refs[g].s];
+    }
+    P.w21.assign(wsize, 0.f);
     P.diag.assign(dsize, 0.f); P.diagT.assign(dsize, 0.f); P.l21.assign(lsize, 0.f);
     // diagonal-block inverses and the sub-diagonal blocks in float: small supernodes one per
     // thread, then the big ones one at a time with the threads over their columns
@@ -186,6 +203,16 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         const std::vector<double>& src = f.L21[s];  // column-major nr x ns
         for (int a = 0; a < nr; ++a)
             for (int j = 0; j < ns; ++j) L[(size_t)a * ns + j] = (float)src[(size_t)j * nr + a];
+        if (!in_top[g] && fused_h[height[g]]) {
+            // W(a, j) = sum_{k >= j} L21(a, k) Linv(k, j), Linv(k, j) = inv[j * ns + k]
+            float* W = &P.w21[P.woff[g]];
+            for (int a = 0; a < nr; ++a)
+                for (int j = 0; j < ns; ++j) {
+                    double acc = 0.0;
+                    for (int k = j; k < ns; ++k) acc += src[(size_t)k * nr + a] * inv[(size_t)j * ns + k];
+                    W[(size_t)a * ns + j] = (float)acc;
+                }
+        }
     };
     constexpr int kBigInverse = 256;
 #pragma omp parallel for schedule(dynamic)
@@ -288,6 +315,8 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         if (contrib_rows > 0) P.Gc[h] = lanes_for(contrib_len / contrib_rows);
     }
     P.sn_begin[nh] = (int)P.sn_list.size();
+    for (int h = 0; h < nh; ++h)
+        if ((bool)P.fused[h] != (bool)fused_h[h]) throw std::runtime_error("triangular-solve plan: fused heights disagree");
     // runs of block-sharing supernodes (sn_list is ordered by block offsets within each height)
     P.run_begin.assign(nh + 1, 0);
     P.run_max.assign(nh, 1);

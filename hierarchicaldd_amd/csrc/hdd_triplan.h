@@ -42,6 +42,10 @@ struct TriPlanHost {
     std::vector<int> sn_of;                   // plan row -> supernode
     std::vector<long long> loff, doff;        // offsets into l21 / diag
     std::vector<float> diag, diagT, l21;
+    // supernodes of fused heights: W_g = L21_g Linv_g (nr x n, row-major like L21, formed
+    // in double), so that the fused kernels need not wait for y (forward) or t (backward)
+    std::vector<long long> woff;              // offsets into w21 (fused heights only)
+    std::vector<float> w21;
     static constexpr int kFusedMinSupernodes = 1024;  // a height is done by one workgroup per supernode only with this many
     std::vector<int> sn_begin, max_sz, big;   // per height: supernode range in sn_list, largest supernode, 1 if > kSliceRows
     std::vector<int> fused;                   // per height: 1 if every supernode is <= kSliceRows and there are many
