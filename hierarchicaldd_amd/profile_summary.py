@@ -17,7 +17,9 @@ kernel and grid size is printed. With --pmc --per-solve N --copy-bytes B
 written, then N solves of one triangular-solve plan) the counter is also
 summed over every dispatch after the last k_copy and divided by N, and the
 copy's mean value is printed beside B, so that the plan's HBM bytes per solve
-are the sum scaled by B over the copy's value.
+are the sum scaled by B over the copy's value. With --per-solve N on a kernel
+trace of the same run, the kernel time and the span (first start to last end)
+per solve after the last k_copy are printed, with each kernel's share.
 """
 from __future__ import annotations
 
@@ -68,6 +70,37 @@ def from_db(path, by_grid=False, first=None):
                 con.execute(f"select name, count(*), sum(duration), avg(duration) from kernels{where} group by name").fetchall()]
     con.close()
     return [(n, int(c), float(t) / 1e3, float(a) / 1e3) for n, c, t, a in rows]  # ns -> us
+
+
+def trace_per_solve(path, n_solves):
+    """Kernel trace of `bench.py --plan-only`: the dispatches after the last k_copy are the N
+    solves of one plan. Per solve: the summed kernel time, the span from the first dispatch's
+    start to the last one's end (what HIP events around the solve measure, launch gaps
+    included), and each kernel's dispatches and time."""
+    con = sqlite3.connect(path)
+    st = start_column(con)
+    rows = con.execute(f"select name, {st}, duration from kernels order by {st}").fetchall()
+    con.close()
+    copies = [i for i, (n, t, d) in enumerate(rows) if short(n).endswith("k_copy")]
+    if not copies:
+        raise SystemExit("no k_copy dispatch: --per-solve needs the calibration copies before the solves")
+    after = rows[copies[-1] + 1:]
+    if not after:
+        raise SystemExit("no dispatch after the last k_copy")
+    total = sum(float(d) for _, _, d in after)
+    span = max(float(t) + float(d) for _, t, d in after) - float(after[0][1])
+    kernels = {}
+    for n, _, d in after:
+        k = kernels.setdefault(short(n), [0, 0.0])
+        k[0] += 1
+        k[1] += float(d)
+    lines = [f"solves: {n_solves}, dispatches after the last copy: {len(after)}",
+             f"kernel time per solve: {total / n_solves / 1e3:.1f} us",
+             f"span per solve (first start to last end, launch gaps included): {span / n_solves / 1e3:.1f} us",
+             f"{'kernel':48s} {'dispatches/solve':>16s} {'us/solve':>10s}"]
+    for n, (c, t) in sorted(kernels.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"{n[:48]:48s} {c / n_solves:16.1f} {t / n_solves / 1e3:10.1f}")
+    return lines
 
 
 def from_csv(path):
@@ -142,7 +175,8 @@ def main():
     ap.add_argument("--top", type=int, default=0, help="print only the N largest rows")
     ap.add_argument("--pmc", action="store_true")
     ap.add_argument("--from", dest="first", default=None, help="count dispatches from the first launch of this kernel on")
-    ap.add_argument("--per-solve", type=int, default=0, help="with --pmc: counter sum per solve after the last k_copy")
+    ap.add_argument("--per-solve", type=int, default=0,
+                    help="counter sum (with --pmc) or kernel time and span (kernel trace) per solve after the last k_copy")
     ap.add_argument("--copy-bytes", type=float, default=0.0, help="with --per-solve: bytes read (and written) by one calibration copy")
     a = ap.parse_args()
     if a.pmc:
@@ -156,6 +190,15 @@ def main():
     csvs = glob.glob(os.path.join(a.dir, "**", "*kernel_stats.csv"), recursive=True)
     if a.first and not dbs:
         raise SystemExit("--from needs the rocpd database output")
+    if a.per_solve:
+        if not dbs:
+            raise SystemExit("--per-solve on a kernel trace needs the rocpd database output")
+        text = "\n".join(([f"command: {a.cmd}"] if a.cmd else []) + trace_per_solve(dbs[0], a.per_solve)) + "\n"
+        if a.out:
+            with open(a.out, "w") as f:
+                f.write(text)
+        sys.stdout.write(text)
+        return
     rows = from_db(dbs[0], a.by_grid, a.first) if dbs else from_csv(csvs[0])
     agg = {}
     for n, c, t, _ in rows:
