@@ -471,7 +471,11 @@ def batched_leg(hdd, count, seed=0):
             "forms_solves_per_s": {k: count / v for k, v in forms.items()},
             "plain_solve_s": plain_s, "batched_solve_s": forms[best], "host_setup_s": setup_s, "setup_wall_s": setup_wall,
             "batch_device_build_s": build_s,
-            "max_iteration_diff": int(max(abs(a - b) for a, b in zip(iters, bit))),
+            "max_iteration_diff": {"one_batch": int(max(abs(a - b) for a, b in zip(iters, bit))),
+                                   "size_grouped_batches_at_once": int(max(abs(iters[p] - git[k]) for k, p in enumerate(perm))),
+                                   "contexts_at_once": int(max(abs(a - b) for a, b in zip(iters, it_c)))},
+            "iterations_off_by_more_than_one": {"one_batch": int(sum(abs(a - b) > 1 for a, b in zip(iters, bit))),
+                                                "size_grouped_batches_at_once": int(sum(abs(iters[p] - git[k]) > 1 for k, p in enumerate(perm)))},
             "contexts_at_once_iterations_equal": bool(it_c == iters),
             "max_x_rel_diff_vs_alone": xdiff, "all_converged": bool(min(bit) >= 0 and min(it_c) >= 0 and min(git) >= 0)}
 
