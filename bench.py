@@ -405,8 +405,14 @@ def batched_leg(hdd, count, seed=0):
     # library releases the interpreter lock, and set-up is not part of either solve rate
     from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
+    share = max(1, len(os.sched_getaffinity(0)) // 4)
+
+    def setup_alone(p):
+        hdd.set_host_threads(share)  # four set-ups side by side, each on its share of the host
+        return hdd.DDSolver(*p)
+
     with ThreadPoolExecutor(max_workers=4) as ex:
-        singles = list(ex.map(lambda p: hdd.DDSolver(*p), probs))
+        singles = list(ex.map(setup_alone, probs))
     setup_wall = time.time() - t0
     plain_s, setup_s, iters, xs, alone_s = 0.0, 0.0, [], [], []
     for (dom, cfg), g in zip(probs, singles):

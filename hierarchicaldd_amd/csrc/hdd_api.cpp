@@ -1,4 +1,5 @@
 // extern "C" boundary of the library (declarations and reference citations in include/hdd.h).
+#include <omp.h>
 #include "../../include/hdd.h"
 #include "hdd_runtime.h"
 #include "hdd_server.h"
@@ -227,6 +228,12 @@ hdd_status hdd_set_vcycle_storage(hdd_ctx* ctx, int fp16) {
 hdd_status hdd_set_dot_order(hdd_ctx* ctx, int serial) {
     if (!ctx) return HDD_ERR_ARG;
     ctx->solver->serial_dots = serial != 0;
+    return HDD_OK;
+}
+
+hdd_status hdd_set_host_threads(int n) {
+    if (n < 0) { g_create_error = "hdd_set_host_threads: n must be >= 0"; return HDD_ERR_ARG; }
+    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
     return HDD_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
+#include <omp.h>
 
 namespace hdd {
 
@@ -61,8 +62,10 @@ ProblemHost::ProblemHost(const SolverParams& p, int n_pages, const uint64_t* pag
     {
         const PagedLevel input = L0;
         std::exception_ptr err;
-        std::thread levels([&] {
+        const int omp_threads = omp_get_max_threads();
+        std::thread levels([&, omp_threads] {
             try {
+                omp_set_num_threads(omp_threads);
                 auto tl = std::chrono::steady_clock::now();
                 divide_subdomains(L0, P.size, P.sub);
                 set_face_flags(L0);

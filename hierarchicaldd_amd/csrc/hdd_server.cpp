@@ -1,4 +1,5 @@
 #include "hdd_server.h"
+#include <omp.h>
 #include <algorithm>
 #include <stdexcept>
 
@@ -12,7 +13,13 @@ Server::Server(const ServerParams& p) : prm(p), t0(std::chrono::steady_clock::no
         hdd_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         budget_ = 0.9 * (double)free_b;
     }
-    for (int i = 0; i < prm.setup_threads; ++i) threads.emplace_back([this] { setup_worker(); });
+    // the set-up workers share the host's threads: each runs its OpenMP work on its share
+    const int per_worker = std::max(1, omp_get_max_threads() / prm.setup_threads);
+    for (int i = 0; i < prm.setup_threads; ++i)
+        threads.emplace_back([this, per_worker] {
+            omp_set_num_threads(per_worker);
+            setup_worker();
+        });
     threads.emplace_back([this] { builder(); });
     threads.emplace_back([this] { driver(); });
 }

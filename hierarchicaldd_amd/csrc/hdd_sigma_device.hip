@@ -1452,7 +1452,12 @@ void factor_sigma(const SigmaHost& H, SigmaFactors& F, bool wait) {
     // the threads use this thread's device (dense fronts of the factorisation may run there)
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) dev = -1;
-    auto same_device = [dev] { if (dev >= 0) (void)hipSetDevice(dev); };
+    // ... and this thread's OpenMP thread count (a server's set-up worker runs on its share)
+    const int omp_threads = omp_get_max_threads();
+    auto same_device = [dev, omp_threads] {
+        if (dev >= 0) (void)hipSetDevice(dev);
+        omp_set_num_threads(omp_threads);
+    };
     auto arr_factor = [&](int l) {
         auto tp = std::chrono::steady_clock::now();
         std::vector<Int3> rc(H.if_coords[l].size());

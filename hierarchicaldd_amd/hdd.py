@@ -59,7 +59,8 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_set_sigma_refinement", "hdd_sigma_factor_solve", "hdd_set_stream", "hdd_synchronize", "hdd_multiply",
            "hdd_inner_product", "hdd_convergence_norm", "hdd_project", "hdd_vector_copy", "hdd_vector_copy2",
            "hdd_vector_plus_equal", "hdd_vector_minus_equal", "hdd_vector_scale", "hdd_apply_preconditioner_channels",
-           "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_set_dot_order", "hdd_inner_product_noalloc", "hdd_batches_pcg_solve"]
+           "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_set_dot_order", "hdd_inner_product_noalloc", "hdd_batches_pcg_solve",
+           "hdd_set_host_threads"]
 
 
 def lib():
@@ -421,6 +422,14 @@ class DDBatch(DDSolver):
         out = np.zeros((self.problem_pages(p), 256), dtype=np.float32)
         self._check(lib().hdd_batch_get_x(self.h, p, _ptr(out)))
         return out
+
+
+def set_host_threads(n: int):
+    """Host threads of the set-ups the calling thread starts from now on (hdd_set_host_threads;
+    0: all the host's)."""
+    st = lib().hdd_set_host_threads(int(n))
+    if st != 0:
+        raise HddError(f"hdd_set_host_threads failed ({st}): {lib().hdd_last_error(None).decode()}")
 
 
 def solve_concurrently(batches, tolerances, max_iterations: int = 5000):

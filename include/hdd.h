@@ -160,6 +160,13 @@ hdd_status hdd_set_sigma_refinement(hdd_ctx* ctx, int steps);
  * checks against the reference's own rounding). */
 hdd_status hdd_set_dot_order(hdd_ctx* ctx, int serial);
 
+/* Host threads of the set-up work (interface generator, factorisations, plan
+ * building) that the calling thread starts from now on: n >= 1 threads, or
+ * n = 0 for all the host's (the default). Per calling thread, so a caller
+ * that sets several problems up side by side can give each its share (the
+ * server's set-up workers take theirs themselves). No reference counterpart. */
+hdd_status hdd_set_host_threads(int n);
+
 /* Precision of the PCG loop (no reference counterpart): fp64 = 0 (default)
  * keeps CONJUGATE_GRADIENT_DUMPING's float rho, s.q, alpha and beta
  * (DDS/CONJUGATE_GRADIENT_DUMPING.cpp:193-205), with its true residual only at

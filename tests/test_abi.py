@@ -46,3 +46,25 @@ def test_setup_failure_reports_error():
     cfg = domain.DDConfig(sub_size=8, vlevels=2, sigma_levels=6)
     with pytest.raises(hdd.HddError, match="sigma_levels"):
         hdd.DDSolver(dom, cfg, host_only=True)
+
+
+def test_host_threads_setting_keeps_setups_equal():
+    """hdd_set_host_threads changes only how many host threads the calling thread's set-ups
+    use: a host-only set-up on two threads gives the interface operators and V-cycle flags of
+    one on all the host's threads; a negative count is an argument error."""
+    from hierarchicaldd_amd import domain, hdd
+    dom, cfg = domain.small_box(32, 16, 3, seed=5)
+    a = hdd.DDSolver(dom, cfg, host_only=True)
+    hdd.set_host_threads(2)
+    try:
+        b = hdd.DDSolver(dom, cfg, host_only=True)
+    finally:
+        hdd.set_host_threads(0)
+    try:
+        for lvl in range(cfg.vlevels):
+            assert np.array_equal(a.level(lvl)[1], b.level(lvl)[1])
+        assert a.stats()["interface_dofs"] == b.stats()["interface_dofs"] > 0
+    finally:
+        a.close()
+        b.close()
+    assert hdd.lib().hdd_set_host_threads(-1) == 1
