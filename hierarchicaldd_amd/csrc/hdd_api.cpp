@@ -1,5 +1,6 @@
 // extern "C" boundary of the library (declarations and reference citations in include/hdd.h).
 #include <omp.h>
+#include <cstdlib>
 #include "../../include/hdd.h"
 #include "hdd_runtime.h"
 #include "hdd_server.h"
@@ -233,7 +234,13 @@ hdd_status hdd_set_dot_order(hdd_ctx* ctx, int serial) {
 
 hdd_status hdd_set_host_threads(int n) {
     if (n < 0) { g_create_error = "hdd_set_host_threads: n must be >= 0"; return HDD_ERR_ARG; }
-    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+    if (n == 0) {
+        // OpenMP's own default: OMP_NUM_THREADS (its first value) if set, else the processors available
+        const char* e = std::getenv("OMP_NUM_THREADS");
+        const int v = e ? std::atoi(e) : 0;
+        n = v > 0 ? v : omp_get_num_procs();
+    }
+    omp_set_num_threads(n);
     return HDD_OK;
 }
 

@@ -17,6 +17,9 @@
 // accumulation of Arr — are replayed serially in subdomain order afterwards,
 // which reproduces the reference's sequential results exactly.
 #include <array>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -487,6 +490,14 @@ Csr rows_to_csr(const std::vector<std::vector<std::pair<int, float>>>& rows, int
 }  // namespace
 
 void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, int mgl, SigmaHost& out) {
+    // phase timings on stderr when HDD_FACTOR_VERBOSE is set
+    static const bool verbose = std::getenv("HDD_FACTOR_VERBOSE") != nullptr;
+    auto tp = std::chrono::steady_clock::now();
+    auto phase = [&](const char* what) {
+        const auto now = std::chrono::steady_clock::now();
+        if (verbose) std::fprintf(stderr, "[setup]   interface generator, %s: %.3fs\n", what, std::chrono::duration<double>(now - tp).count());
+        tp = now;
+    };
     const Int3 ns = subdomain_counts(size, sub);
     const int nsub = ns.x * ns.y * ns.z;
     std::vector<Subdomain> subs(nsub);
@@ -503,6 +514,7 @@ void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, 
             build_hierarchy(S, mgl);
         });
     errs.rethrow();
+    phase("split and hierarchy");
     // phase 2 (serial, subdomain order): global interface numbering per level
     out.levels = mgl; out.nsub = nsub;
     out.n_if.assign(mgl, 0);
@@ -519,10 +531,12 @@ void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, 
                     if (l == 0) out.interface_map.push_back(c);
                 }
             }
+    phase("interface numbering");
     // phase 3 (parallel): Galerkin assembly per subdomain
 #pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < nsub; ++s) errs.run([&] { assemble(subs[s], mgl, if_id); });
     errs.rethrow();
+    phase("Galerkin assembly");
     // phase 4 (serial, subdomain order): accumulate Arr in the reference's order
     std::vector<std::vector<std::vector<std::pair<int, float>>>> arr(mgl);
     for (int l = 0; l < mgl; ++l) arr[l].assign(out.n_if[l], {});
@@ -539,6 +553,7 @@ void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, 
     out.Aii.assign(mgl, std::vector<Csr>(nsub));
     out.Air.assign(mgl, std::vector<Csr>(nsub));
     out.Aii_coords.assign(mgl, std::vector<std::vector<Int3>>(nsub));
+    phase("Arr accumulation");
     for (int l = 0; l < mgl; ++l) out.Arr[l] = rows_to_csr(arr[l], out.n_if[l]);
 #pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < nsub; ++s)
@@ -553,6 +568,7 @@ void generate_interface_operators(const PagedLevel& input, Int3 size, Int3 sub, 
             for (int k = out.Arr[0].ptr[i]; k < out.Arr[0].ptr[i + 1]; ++k) diag |= out.Arr[0].idx[k] == i;
             if (!diag) throw std::runtime_error("interface operator has a zero diagonal entry");
         }
+    phase("operators to CSR");
     // Prr: trilinear prolongation between consecutive interface levels
     out.Prr.assign(std::max(0, mgl - 1), {});
     for (int l = 0; l + 1 < mgl; ++l) {
