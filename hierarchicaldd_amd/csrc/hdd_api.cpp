@@ -204,6 +204,8 @@ hdd_status hdd_batches_pcg_solve(hdd_ctx* const* ctxs, int count, const hdd_pcg_
     } catch (std::exception& e) {
         if (c < count) ctxs[c]->err = e.what();
         g_create_error = e.what();
+        // leave no iteration of any context in flight (its scalars land in the context's pinned buffers)
+        for (int d = 0; d < count; ++d) (void)hipStreamSynchronize(ctxs[d]->solver->stream);
         return HDD_ERR_DEVICE;
     }
 }
