@@ -224,7 +224,10 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
     const int nthreads = omp_get_max_threads();
     const bool use_device = device_fronts_enabled();
     const int front_min = device_front_min_rows();
-    auto front = [&](int s, bool inner_parallel, std::vector<int>& pos) {
+    // which fronts go to the GPU (whose rounding differs from the host's) must not depend on
+    // the thread count: big fronts of heights with fewer than kDeviceHeightFronts fronts
+    constexpr int kDeviceHeightFronts = 16;
+    auto front = [&](int s, bool inner_parallel, bool device_ok, std::vector<int>& pos) {
         const Supernode& S = F.sn[s];
         const int nS = S.n, nR = (int)S.rows.size(), m = nS + nR;
         std::vector<double> Fm((size_t)m * m, 0.0);
@@ -249,7 +252,7 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
             }
             std::vector<double>().swap(U[c]);
         }
-        if (inner_parallel && m >= front_min && use_device) partial_cholesky_device(Fm.data(), m, nS);
+        if (device_ok && m >= front_min && use_device) partial_cholesky_device(Fm.data(), m, nS);
         else partial_cholesky(Fm.data(), m, nS, inner_parallel);
         std::vector<double> l11((size_t)nS * nS), l21((size_t)nR * nS);
         for (int j = 0; j < nS; ++j) {
@@ -267,18 +270,19 @@ void factor_spd(const Csr& A, const std::vector<Int3>& coords, double eps, Super
     for (int h = 0; h <= F.max_height; ++h) {
         const auto& list = by_h[h];
         auto th0 = std::chrono::steady_clock::now();
+        const bool device_ok = (int)list.size() < kDeviceHeightFronts;
         if ((int)list.size() >= nthreads) {
             ParallelErrors errs;
 #pragma omp parallel
             {
                 std::vector<int> pos(n);
 #pragma omp for schedule(dynamic)
-                for (size_t k = 0; k < list.size(); ++k) errs.run([&] { front(list[k], false, pos); });
+                for (size_t k = 0; k < list.size(); ++k) errs.run([&] { front(list[k], false, device_ok, pos); });
             }
             errs.rethrow();
         } else {
             std::vector<int> pos(n);
-            for (int s : list) front(s, true, pos);
+            for (int s : list) front(s, true, device_ok, pos);
         }
         if (verbose && n > 20000) {
             size_t mmax = 0; double flops = 0;

@@ -11,6 +11,11 @@
 
 namespace hdd {
 
+// The host block solves and inner products of the bottom spectrum split their work
+// as for this many threads, whichever threads run it, so that their sums (and the
+// modes) are the same on any OpenMP thread count.
+constexpr int kSpectralParts = 16;
+
 void sym_eig(int n, std::vector<double>& A, std::vector<double>& w) {
     w.assign(n, 0.0);
     if (n == 0) return;
@@ -311,8 +316,9 @@ ParallelSolve8::ParallelSolve8(const SupernodalFactor& F_) : F(F_) {
         task_end.push_back(F.sn[s].s0 + F.sn[s].n);
     };
     if (!postorder) { add_task(ns - 1); task_first.back() = 0; return; }  // one serial task over everything
-    const int threads = std::max(1, omp_get_max_threads());
-    const size_t target = (size_t)4 * threads;
+    // subtrees for kSpectralParts threads, whatever runs them: the split, and so the order
+    // in which the subtrees' buffers are summed, does not depend on the thread count
+    const size_t target = (size_t)4 * kSpectralParts;
     double total = 0;
     std::vector<int> heap;
     for (int s = 0; s < ns; ++s) if (F.sn[s].parent < 0) { heap.push_back(s); total += work[s]; }
@@ -444,14 +450,17 @@ void solve_schur(const ParallelSolve8& Fs, const SupernodalFactor& Fc, int n_int
 
 // C (cols x p) = A[:, :cols]^T W over rows, A and W column-major with leading dimension n
 void gemm_tn(int n, int cols, int p, const double* A, const double* W, double* C) {
-    const int nt = omp_get_max_threads();
+    // partial sums over kSpectralParts contiguous ranges of 1024-row chunks (the ranges a static
+    // schedule gives that many threads), added in range order: the same sums on any thread count
+    const int nt = kSpectralParts;
+    const int nchunks = (n + 1023) / 1024, q = nchunks / nt, rem = nchunks % nt;
     std::vector<double> part((size_t)nt * cols * p, 0.0);
-#pragma omp parallel
-    {
-        double* P = &part[(size_t)omp_get_thread_num() * cols * p];
-#pragma omp for schedule(static)
-        for (int r0 = 0; r0 < n; r0 += 1024) {
-            const int r1 = std::min(n, r0 + 1024);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < nt; ++t) {
+        double* P = &part[(size_t)t * cols * p];
+        const int k0 = t * q + std::min(t, rem), k1 = k0 + q + (t < rem ? 1 : 0);
+        for (int k = k0; k < k1; ++k) {
+            const int r0 = k * 1024, r1 = std::min(n, r0 + 1024);
             for (int c = 0; c < cols; ++c) {
                 const double* ac = A + (size_t)c * n;
                 for (int a = 0; a < p; ++a) {

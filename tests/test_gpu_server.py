@@ -150,3 +150,27 @@ def test_server_continuous_batching_and_cancel():
         joined += rep["concurrent"] >= 2
     # long_b was mid-solve (it never converges) while every probe ran
     assert joined == len(probes)
+
+
+def test_served_request_bits_on_a_worker_share():
+    """A request set up by one of four set-up workers (each on a quarter of the host's
+    threads) gives the bits of its solve alone (set up on all of them): the set-up does
+    not depend on the thread count, down to the bottom spectrum's block Lanczos sums and
+    which factorisation fronts run on the GPU."""
+    from hierarchicaldd_amd import hdd
+    from tests.gpu_common import same_bits
+    dom, cfg = domain.sphere_case(96, 32, 4, delta=(32, 32, 32))
+    g = hdd.DDSolver(dom, cfg)
+    try:
+        it_a, _, _ = g.solve(_tol(dom), 300)
+        x_a = g.get(hdd.CH_X)
+    finally:
+        g.close()
+    srv = hdd.Server(setup_threads=4, groups=1, max_batch=1)
+    try:
+        st, x, rep = srv.wait(srv.submit(dom, cfg, _tol(dom), 300))
+    finally:
+        srv.close()
+    print(f"[worker share] alone {it_a} served {rep['iterations']}, bit-identical {same_bits(x, x_a)}", flush=True)
+    assert st == 0 and rep["iterations"] == it_a
+    assert same_bits(x, x_a)

@@ -113,3 +113,24 @@ def test_rejects_bad_cut(case):
     for bad in (0.0, 1.0, -0.5):
         with pytest.raises(hdd.HddError):
             g.bottom_spectrum(bad)
+
+
+def test_same_modes_on_any_thread_count():
+    """The block Lanczos path splits its block solves and inner products as for a fixed
+    number of threads, so the modes are the same bits on any OpenMP thread count (a set-up
+    run by a server's worker on its share of the host gives the modes of a set-up alone;
+    before, 8 and 3 threads gave eigenvalues 1e-15 apart on this case)."""
+    from hierarchicaldd_amd import hdd
+    dom, cfg = domain.sphere_case(96, 32, 4, delta=(32, 32, 32))
+    g = hdd.DDSolver(dom, cfg, host_only=True)
+    try:
+        ref = g.bottom_spectrum(0.5)
+        for n in (1, 3):
+            hdd.set_host_threads(n)
+            try:
+                got = g.bottom_spectrum(0.5)
+            finally:
+                hdd.set_host_threads(0)
+            assert np.array_equal(ref[0], got[0]) and ref[1] == got[1], n
+    finally:
+        g.close()
