@@ -86,15 +86,14 @@ __global__ void __launch_bounds__(256) k_tri_fwd_reduce(const int* __restrict__ 
 // big heights, else summed here), then y[r] = sum_{k<=r} Linv(r,k) t[k] with
 // G lanes per row.
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
-                                                    const int* __restrict__ s0, const int* __restrict__ sz,
-                                                    const long long* __restrict__ doff, const float* __restrict__ diag,
-                                                    const float* __restrict__ tg, const int* __restrict__ inc_ptr,
-                                                    const long long* __restrict__ cpos, const float* __restrict__ c,
-                                                    const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ y,
-                                                    const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    extern __shared__ float ts[];
-    const int g = sl_sn[q0 + blockIdx.x], r0 = sl_r0[q0 + blockIdx.x], n = sz[g], base = s0[g];
+__device__ __forceinline__ void fwd_sn_slice(int q, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
+                                             const int* __restrict__ s0, const int* __restrict__ sz,
+                                             const long long* __restrict__ doff, const float* __restrict__ diag,
+                                             const float* __restrict__ tg, const int* __restrict__ inc_ptr,
+                                             const long long* __restrict__ cpos, const float* __restrict__ c,
+                                             const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ y,
+                                             const int* __restrict__ sn_prob, const int* __restrict__ active, float* ts) {
+    const int g = sl_sn[q], r0 = sl_r0[q], n = sz[g], base = s0[g];
     if (sn_stopped(sn_prob, active, g)) return;
     const int rend = min(n, r0 + slice_rows);
     for (int k = threadIdx.x; k < rend; k += 256) {
@@ -121,20 +120,31 @@ __global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, cons
     }
 }
 
-// contributions c_g[a] = L21_g[a, :] . y_g for the rows [a0, a1) of an item, G lanes per row
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_fwd_contrib(int i0, int i1, const int* __restrict__ ci_sn, const int* __restrict__ ci_a0,
-                                                         const int* __restrict__ ci_a1, const int* __restrict__ s0, const int* __restrict__ sz,
-                                                         const long long* __restrict__ loff, const long long* __restrict__ cofs,
-                                                         const float* __restrict__ l21, const float* __restrict__ y, float* __restrict__ c,
-                                                         const int* __restrict__ sn_prob, const int* __restrict__ active) {
+__global__ void __launch_bounds__(256) k_tri_fwd_sn(int q0, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
+                                                    const int* __restrict__ s0, const int* __restrict__ sz,
+                                                    const long long* __restrict__ doff, const float* __restrict__ diag,
+                                                    const float* __restrict__ tg, const int* __restrict__ inc_ptr,
+                                                    const long long* __restrict__ cpos, const float* __restrict__ c,
+                                                    const int* __restrict__ orig_of, const float* __restrict__ b, float* __restrict__ y,
+                                                    const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    extern __shared__ float ts[];
+    fwd_sn_slice<G>(q0 + blockIdx.x, slice_rows, sl_sn, sl_r0, s0, sz, doff, diag, tg, inc_ptr, cpos, c, orig_of, b, y, sn_prob, active, ts);
+}
+
+// contributions c_g[a] = L_g[a, :] . v_g for the rows [a0, a1) of an item, G lanes per row:
+// L21 and v = y, or (W form) W = L21 Linv and v = t
+template <int G>
+__device__ __forceinline__ void fwd_contrib_item(int it, const int* __restrict__ ci_sn, const int* __restrict__ ci_a0,
+                                                 const int* __restrict__ ci_a1, const int* __restrict__ s0, const int* __restrict__ sz,
+                                                 const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                 const float* __restrict__ l21, const float* __restrict__ v, float* __restrict__ c,
+                                                 const int* __restrict__ sn_prob, const int* __restrict__ active) {
     // a workgroup per item; its rows go to the 256/G lane groups in turn
-    const int it = i0 + blockIdx.x;
-    if (it >= i1) return;
     const int g = ci_sn[it], n = sz[g], a1 = ci_a1[it];
     if (sn_stopped(sn_prob, active, g)) return;
     const float* L = l21 + loff[g];
-    const float* yy = y + s0[g];
+    const float* yy = v + s0[g];
     float* cg = c + cofs[g];
     const int lane = threadIdx.x & (G - 1);
     for (int a = ci_a0[it] + (int)threadIdx.x / G; a < a1; a += 256 / G) {
@@ -147,21 +157,52 @@ __global__ void __launch_bounds__(256) k_tri_fwd_contrib(int i0, int i1, const i
     }
 }
 
-// backward tiles: one wavefront per (supernode g, row chunk cc, column block j0);
-// lane j accumulates L21_g[a, j] z[rows_g[a]] over the chunk's rows (z: the
-// ancestors' backward results, final)
-__global__ void __launch_bounds__(256) k_tri_bwd_tiles(int i0, int i1, const int* __restrict__ bi_sn, const int* __restrict__ bi_c,
-                                                       const int* __restrict__ bi_j0, const int* __restrict__ sz, const int* __restrict__ nr,
-                                                       const long long* __restrict__ loff, const int* __restrict__ roff,
-                                                       const int* __restrict__ srows, const long long* __restrict__ bofs,
-                                                       const float* __restrict__ l21, const float* __restrict__ z, float* __restrict__ bpart,
-                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    const int it = i0 + blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_fwd_contrib(int i0, int i1, const int* __restrict__ ci_sn, const int* __restrict__ ci_a0,
+                                                         const int* __restrict__ ci_a1, const int* __restrict__ s0, const int* __restrict__ sz,
+                                                         const long long* __restrict__ loff, const long long* __restrict__ cofs,
+                                                         const float* __restrict__ l21, const float* __restrict__ y, float* __restrict__ c,
+                                                         const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    const int it = i0 + blockIdx.x;
     if (it >= i1) return;
+    fwd_contrib_item<G>(it, ci_sn, ci_a0, ci_a1, s0, sz, loff, cofs, l21, y, c, sn_prob, active);
+}
+
+// W form of a height with big supernodes, one launch after the reduce kernel's t:
+// workgroups [0, nq) are the diagonal step's slices (y = Linv t), the others the
+// contribution items (c = W t); neither waits for the other.
+template <int G1, int G2>
+__global__ void __launch_bounds__(256) k_tri_fwd_sn_contrib_w(int q0, int nq, int slice_rows, const int* __restrict__ sl_sn,
+                                                              const int* __restrict__ sl_r0, const int* __restrict__ s0,
+                                                              const int* __restrict__ sz, const long long* __restrict__ doff,
+                                                              const float* __restrict__ diag, const float* __restrict__ t,
+                                                              float* __restrict__ y, int i0, const int* __restrict__ ci_sn,
+                                                              const int* __restrict__ ci_a0, const int* __restrict__ ci_a1,
+                                                              const long long* __restrict__ woff, const long long* __restrict__ cofs,
+                                                              const float* __restrict__ w21, float* __restrict__ c,
+                                                              const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    extern __shared__ float ts[];
+    if ((int)blockIdx.x < nq)
+        fwd_sn_slice<G1>(q0 + blockIdx.x, slice_rows, sl_sn, sl_r0, s0, sz, doff, diag, t, nullptr, nullptr, nullptr, nullptr, nullptr, y,
+                         sn_prob, active, ts);
+    else
+        fwd_contrib_item<G2>(i0 + (int)blockIdx.x - nq, ci_sn, ci_a0, ci_a1, s0, sz, woff, cofs, w21, t, c, sn_prob, active);
+}
+
+// backward tiles: one wavefront per (supernode g, row chunk cc, column block j0);
+// lane j accumulates L_g[a, j] z[rows_g[a]] over the chunk's rows (z: the
+// ancestors' backward results, final; L: L21, or W in the W form)
+__device__ __forceinline__ void bwd_tile_item(int it, const int* __restrict__ bi_sn, const int* __restrict__ bi_c,
+                                              const int* __restrict__ bi_j0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                              const long long* __restrict__ loff, const int* __restrict__ roff,
+                                              const int* __restrict__ srows, const long long* __restrict__ bofs,
+                                              const float* __restrict__ l21, const float* __restrict__ z, float* __restrict__ bpart,
+                                              const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    const int lane = threadIdx.x & 63;
     const int g = bi_sn[it], n = sz[g], cc = bi_c[it], j = bi_j0[it] + lane;
     if (sn_stopped(sn_prob, active, g)) return;
     const int a0 = cc * TriPlanHost::kTile, m = min(nr[g] - a0, TriPlanHost::kTile);
-    // lane q holds z at the tile's row q; all the tile's L21 loads are issued
+    // lane q holds z at the tile's row q; all the tile's loads are issued
     // before the first product (one memory round trip), then summed in row order
     const float zq = lane < m ? z[srows[roff[g] + a0 + lane]] : 0.f;
     const float* L = l21 + loff[g] + (long long)a0 * n + min(j, n - 1);
@@ -175,8 +216,21 @@ __global__ void __launch_bounds__(256) k_tri_bwd_tiles(int i0, int i1, const int
     if (j < n) bpart[bofs[g] + (long long)cc * n + j] = acc;
 }
 
+__global__ void __launch_bounds__(256) k_tri_bwd_tiles(int i0, int i1, const int* __restrict__ bi_sn, const int* __restrict__ bi_c,
+                                                       const int* __restrict__ bi_j0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                                       const long long* __restrict__ loff, const int* __restrict__ roff,
+                                                       const int* __restrict__ srows, const long long* __restrict__ bofs,
+                                                       const float* __restrict__ l21, const float* __restrict__ z, float* __restrict__ bpart,
+                                                       const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    const int it = i0 + blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (it >= i1) return;
+    bwd_tile_item(it, bi_sn, bi_c, bi_j0, sz, nr, loff, roff, srows, bofs, l21, z, bpart, sn_prob, active);
+}
+
 // backward reduce (heights with big supernodes): t_j = y_j - sum over the row
-// chunks of the tile partials of column j of its supernode, G lanes per row
+// chunks of the tile partials of column j of its supernode, G lanes per row.
+// W form (W set): v_j = t_j - (the partials), where t holds Linv^T y, is the
+// result: written to z and to x in the original numbering.
 template <int G>
 __global__ void __launch_bounds__(256) k_tri_bwd_reduce(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
                                                         const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
@@ -196,21 +250,46 @@ __global__ void __launch_bounds__(256) k_tri_bwd_reduce(const int* __restrict__ 
     if (lane == 0) t[i] = y[i] - s;
 }
 
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_bwd_reduce_w(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of,
+                                                          const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                                          const long long* __restrict__ bofs, const float* __restrict__ bpart,
+                                                          const float* __restrict__ t, const int* __restrict__ orig_of,
+                                                          float* __restrict__ z, float* __restrict__ x,
+                                                          const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    const int w = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
+    if (w >= nrows) return;
+    const int i = rows[w], g = sn_of[i], n = sz[g], k = i - s0[g];
+    if (sn_stopped(sn_prob, active, g)) return;
+    const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
+    const float* P = bpart + bofs[g] + k;
+    float s = 0.f;
+#pragma unroll 8
+    for (int cc = lane; cc < nch; cc += G) s += P[(long long)cc * n];
+    s = group_sum<G>(s);
+    if (lane == 0) {
+        const float v = t[i] - s;
+        z[i] = v;
+        x[orig_of[i]] = v;
+    }
+}
+
 // backward diagonal step: t of rows [r0, n) into LDS (reduce output on big
 // heights, else summed here), then z[r] = sum_{k>=r} Linv(k,r) t[k]; y holds
 // the forward result and z the backward one (separate arrays: the slices of
 // one supernode read rows that other slices write); also writes x in the
-// original numbering
+// original numbering. With zout set (W form) the input is y itself and the
+// result, Linv^T y, goes to zout only.
 template <int G>
-__global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
-                                                    const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
-                                                    const long long* __restrict__ doff, const float* __restrict__ diagT,
-                                                    const float* __restrict__ tg, const long long* __restrict__ bofs,
-                                                    const float* __restrict__ bpart, const int* __restrict__ orig_of,
-                                                    const float* __restrict__ y, float* __restrict__ z, float* __restrict__ x,
-                                                    const int* __restrict__ sn_prob, const int* __restrict__ active) {
-    extern __shared__ float ts[];
-    const int g = sl_sn[q0 + blockIdx.x], r0 = sl_r0[q0 + blockIdx.x], n = sz[g], base = s0[g];
+__device__ __forceinline__ void bwd_sn_slice(int q, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
+                                             const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                             const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                             const float* __restrict__ tg, const long long* __restrict__ bofs,
+                                             const float* __restrict__ bpart, const int* __restrict__ orig_of,
+                                             const float* __restrict__ y, float* __restrict__ z, float* __restrict__ x,
+                                             float* __restrict__ zout, const int* __restrict__ sn_prob, const int* __restrict__ active,
+                                             float* ts) {
+    const int g = sl_sn[q], r0 = sl_r0[q], n = sz[g], base = s0[g];
     if (sn_stopped(sn_prob, active, g)) return;
     const int rend = min(n, r0 + slice_rows);
     const int nch = (nr[g] + TriPlanHost::kTile - 1) / TriPlanHost::kTile;
@@ -234,8 +313,54 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, cons
 #pragma unroll 8
         for (int k = r + lane; k < n; k += G) acc += Dr[k] * ts[k];
         acc = group_sum<G>(acc);
-        if (lane == 0) { z[base + r] = acc; x[orig_of[base + r]] = acc; }
+        if (lane == 0) {
+            if (zout) {
+                zout[base + r] = acc;
+            } else {
+                z[base + r] = acc;
+                x[orig_of[base + r]] = acc;
+            }
+        }
     }
+}
+
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_bwd_sn(int q0, int slice_rows, const int* __restrict__ sl_sn, const int* __restrict__ sl_r0,
+                                                    const int* __restrict__ s0, const int* __restrict__ sz, const int* __restrict__ nr,
+                                                    const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                    const float* __restrict__ tg, const long long* __restrict__ bofs,
+                                                    const float* __restrict__ bpart, const int* __restrict__ orig_of,
+                                                    const float* __restrict__ y, float* __restrict__ z, float* __restrict__ x,
+                                                    const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    extern __shared__ float ts[];
+    bwd_sn_slice<G>(q0 + blockIdx.x, slice_rows, sl_sn, sl_r0, s0, sz, nr, doff, diagT, tg, bofs, bpart, orig_of, y, z, x, nullptr,
+                    sn_prob, active, ts);
+}
+
+// W form of a height with big supernodes, backward, one launch before the W reduce:
+// workgroups [0, nq) are the diagonal step's slices on y (Linv^T y into zout), the
+// others the tiles of W^T z (four per workgroup); neither waits for the other.
+template <int G>
+__global__ void __launch_bounds__(256) k_tri_bwd_sn_tiles_w(int q0, int nq, int slice_rows, const int* __restrict__ sl_sn,
+                                                            const int* __restrict__ sl_r0, const int* __restrict__ s0,
+                                                            const int* __restrict__ sz, const int* __restrict__ nr,
+                                                            const long long* __restrict__ doff, const float* __restrict__ diagT,
+                                                            const float* __restrict__ y, float* __restrict__ zout, int i0, int i1,
+                                                            const int* __restrict__ bi_sn, const int* __restrict__ bi_c,
+                                                            const int* __restrict__ bi_j0, const long long* __restrict__ woff,
+                                                            const int* __restrict__ roff, const int* __restrict__ srows,
+                                                            const long long* __restrict__ bofs, const float* __restrict__ w21,
+                                                            const float* __restrict__ z, float* __restrict__ bpart,
+                                                            const int* __restrict__ sn_prob, const int* __restrict__ active) {
+    extern __shared__ float ts[];
+    if ((int)blockIdx.x < nq) {
+        bwd_sn_slice<G>(q0 + blockIdx.x, slice_rows, sl_sn, sl_r0, s0, sz, nr, doff, diagT, y, bofs, nullptr, nullptr, nullptr, nullptr,
+                        nullptr, zout, sn_prob, active, ts);
+        return;
+    }
+    const int it = i0 + ((int)blockIdx.x - nq) * 4 + (int)(threadIdx.x >> 6);
+    if (it >= i1) return;
+    bwd_tile_item(it, bi_sn, bi_c, bi_j0, sz, nr, woff, roff, srows, bofs, w21, z, bpart, sn_prob, active);
 }
 
 // Heights whose supernodes all have <= kSliceRows (128) rows: one workgroup per
@@ -1132,6 +1257,28 @@ static constexpr int kRunM = TriPlanHost::kRunMembers;
         default: hipLaunchKernelGGL(KERNEL<64>, dim3(blocks_for((ROWS) * 64, 256)), dim3(256), 0, __VA_ARGS__); break; \
     }
 
+// k_tri_fwd_sn_contrib_w with the diagonal step's and the contributions' lanes per row
+template <int G1, class... A>
+static void launch_fwd_w_g2(int g2, unsigned grid, size_t shmem, hipStream_t s, A... a) {
+    switch (g2) {
+        case 4: hipLaunchKernelGGL((k_tri_fwd_sn_contrib_w<G1, 4>), dim3(grid), dim3(256), shmem, s, a...); break;
+        case 8: hipLaunchKernelGGL((k_tri_fwd_sn_contrib_w<G1, 8>), dim3(grid), dim3(256), shmem, s, a...); break;
+        case 16: hipLaunchKernelGGL((k_tri_fwd_sn_contrib_w<G1, 16>), dim3(grid), dim3(256), shmem, s, a...); break;
+        case 32: hipLaunchKernelGGL((k_tri_fwd_sn_contrib_w<G1, 32>), dim3(grid), dim3(256), shmem, s, a...); break;
+        default: hipLaunchKernelGGL((k_tri_fwd_sn_contrib_w<G1, 64>), dim3(grid), dim3(256), shmem, s, a...); break;
+    }
+}
+template <class... A>
+static void launch_fwd_w(int g1, int g2, unsigned grid, size_t shmem, hipStream_t s, A... a) {
+    switch (g1) {
+        case 4: launch_fwd_w_g2<4>(g2, grid, shmem, s, a...); break;
+        case 8: launch_fwd_w_g2<8>(g2, grid, shmem, s, a...); break;
+        case 16: launch_fwd_w_g2<16>(g2, grid, shmem, s, a...); break;
+        case 32: launch_fwd_w_g2<32>(g2, grid, shmem, s, a...); break;
+        default: launch_fwd_w_g2<64>(g2, grid, shmem, s, a...); break;
+    }
+}
+
 void TriPlan::fwd_height(int h, const float* b, hipStream_t s) {
     float* yf = y.p;
     const int* SP = sn_prob.p;
@@ -1168,6 +1315,16 @@ void TriPlan::fwd_height(int h, const float* b, hipStream_t s) {
             HDD_ROWS_DISPATCH(Gf[h], k_tri_fwd_reduce, nrows, s, rows, nrows, (const int*)inc_ptr.p, (const long long*)cpos.p,
                               (const float*)c.p, (const int*)orig_of.p, b, t.p, (const int*)sn_of.p, SP, active);
         const int q0 = sl_begin[h], nq = sl_begin[h + 1] - sl_begin[h];
+        if (big[h] && use_w()) {
+            // W form: the slices (y = Linv t) and the contribution items (c = W t) in one launch
+            const int i0 = ci_begin[h], ni = ci_begin[h + 1] - ci_begin[h];
+            if (nq + ni)
+                launch_fwd_w(G[h], Gc[h], (unsigned)(nq + ni), (size_t)max_sz[h] * sizeof(float), s, q0, nq, sl_rows[h], (const int*)sl_sn.p,
+                             (const int*)sl_r0.p, (const int*)s0.p, (const int*)sz.p, (const long long*)doff.p, (const float*)diag.p,
+                             (const float*)t.p, yf, i0, (const int*)ci_sn.p, (const int*)ci_a0.p, (const int*)ci_a1.p,
+                             (const long long*)woff.p, (const long long*)cofs.p, (const float*)w21.p, c.p, SP, active);
+            return;
+        }
         if (nq)
             HDD_SN_DISPATCH(G[h], k_tri_fwd_sn, nq, (size_t)max_sz[h] * sizeof(float), s, q0, sl_rows[h], (const int*)sl_sn.p, (const int*)sl_r0.p,
                             (const int*)s0.p, (const int*)sz.p, (const long long*)doff.p, (const float*)diag.p,
@@ -1212,6 +1369,25 @@ void TriPlan::bwd_height(int h, float* x, hipStream_t s) {
             return;
         }
         const int i0 = bi_begin[h], ni = bi_begin[h + 1] - bi_begin[h];
+        if (big[h] && use_w()) {
+            // W form: the slices (Linv^T y into t) and the tiles of W^T z in one launch, then
+            // z = t - (the tile partials) row by row
+            const int q0 = sl_begin[h], nq = sl_begin[h + 1] - sl_begin[h];
+            const int grid = nq + (int)blocks_for(ni, 4);
+            if (grid)
+                HDD_SN_DISPATCH(G[h], k_tri_bwd_sn_tiles_w, grid, (size_t)max_sz[h] * sizeof(float), s, q0, nq, sl_rows[h], (const int*)sl_sn.p,
+                                (const int*)sl_r0.p, (const int*)s0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)doff.p,
+                                (const float*)diagT.p, (const float*)yf, t.p, i0, i0 + ni, (const int*)bi_sn.p, (const int*)bi_c.p,
+                                (const int*)bi_j0.p, (const long long*)woff.p, (const int*)roff.p, (const int*)srows.p, (const long long*)bofs.p,
+                                (const float*)w21.p, (const float*)zb, bpart.p, SP, active);
+            const int nrows = row_begin[h + 1] - row_begin[h];
+            const int* rows = rowlist.p + row_begin[h];
+            if (nrows)
+                HDD_ROWS_DISPATCH(Gb[h], k_tri_bwd_reduce_w, nrows, s, rows, nrows, (const int*)sn_of.p, (const int*)s0.p, (const int*)sz.p,
+                                  (const int*)nr.p, (const long long*)bofs.p, (const float*)bpart.p, (const float*)t.p, (const int*)orig_of.p,
+                                  zb, x, SP, active);
+            return;
+        }
         if (ni)
             hipLaunchKernelGGL(k_tri_bwd_tiles, dim3(blocks_for(ni, 4)), dim3(256), 0, s, i0, i0 + ni, (const int*)bi_sn.p, (const int*)bi_c.p,
                                (const int*)bi_j0.p, (const int*)sz.p, (const int*)nr.p, (const long long*)loff.p, (const int*)roff.p,

@@ -167,19 +167,23 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         }
     }
     const int nsn = P.nsn = (int)P.s0.size();
-    // heights done by the fused kernels (the same rule as P.fused below): their supernodes get W
-    std::vector<char> fused_h(P.H + 1, 0);
+    // heights done by the fused kernels and heights with big supernodes (the rules of P.fused
+    // and P.big below): their supernodes get W
+    std::vector<char> fused_h(P.H + 1, 0), big_h(P.H + 1, 0), w_h(P.H + 1, 0);
     {
         std::vector<int> count(P.H + 1, 0), mx(P.H + 1, 0);
         for (int g = 0; g < nsn; ++g)
             if (!in_top[g]) { ++count[height[g]]; mx[height[g]] = std::max(mx[height[g]], P.sz[g]); }
-        for (int h = 0; h <= P.H; ++h)
-            fused_h[h] = mx[h] <= TriPlanHost::kSliceRows && count[h] >= TriPlanHost::kFusedMinSupernodes;
+        for (int h = 0; h <= P.H; ++h) {
+            big_h[h] = mx[h] > TriPlanHost::kSliceRows;
+            fused_h[h] = !big_h[h] && count[h] >= TriPlanHost::kFusedMinSupernodes;
+            w_h[h] = fused_h[h] || big_h[h];
+        }
     }
     P.woff.assign(nsn, 0);
     long long wsize = 0;
     for (int g = 0; g < nsn; ++g) {
-        if (in_top[g] || !fused_h[height[g]]) continue;
+        if (in_top[g] || !w_h[height[g]]) continue;
         if (owner[g]) { P.woff[g] = wsize; wsize += (long long)P.sz[g] * P.nr[g]; }
         else P.woff[g] = P.woff[first_gid.at(F[refs[g].m]) + refs[g].s];
     }
@@ -187,6 +191,7 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     P.diag.assign(dsize, 0.f); P.diagT.assign(dsize, 0.f); P.l21.assign(lsize, 0.f);
     // diagonal-block inverses and the sub-diagonal blocks in float: small supernodes one per
     // thread, then the big ones one at a time with the threads over their columns
+    constexpr int kBigInverse = 256;
     auto fill_blocks = [&](int g) {
         const SupernodalFactor& f = *F[refs[g].m];
         const int s = refs[g].s, ns = P.sz[g], nr = P.nr[g];
@@ -203,18 +208,24 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
         const std::vector<double>& src = f.L21[s];  // column-major nr x ns
         for (int a = 0; a < nr; ++a)
             for (int j = 0; j < ns; ++j) L[(size_t)a * ns + j] = (float)src[(size_t)j * nr + a];
-        if (!in_top[g] && fused_h[height[g]]) {
-            // W(a, j) = sum_{k >= j} L21(a, k) Linv(k, j), Linv(k, j) = inv[j * ns + k]
+        if (!in_top[g] && w_h[height[g]]) {
+            // W(a, j) = sum_{k >= j} L21(a, k) Linv(k, j), Linv(k, j) = inv[j * ns + k]; rows of
+            // L21 gathered first (row-major) so the inner loop is contiguous; threads over the
+            // rows for the big supernodes (done one at a time)
             float* W = &P.w21[P.woff[g]];
-            for (int a = 0; a < nr; ++a)
+#pragma omp parallel for schedule(dynamic, 16) if (ns > kBigInverse && !omp_in_parallel())
+            for (int a = 0; a < nr; ++a) {
+                std::vector<double> la(ns);
+                for (int k = 0; k < ns; ++k) la[k] = src[(size_t)k * nr + a];
                 for (int j = 0; j < ns; ++j) {
+                    const double* ij = &inv[(size_t)j * ns];
                     double acc = 0.0;
-                    for (int k = j; k < ns; ++k) acc += src[(size_t)k * nr + a] * inv[(size_t)j * ns + k];
+                    for (int k = j; k < ns; ++k) acc += la[k] * ij[k];
                     W[(size_t)a * ns + j] = (float)acc;
                 }
+            }
         }
     };
-    constexpr int kBigInverse = 256;
 #pragma omp parallel for schedule(dynamic)
     for (int g = 0; g < nsn; ++g)
         if (owner[g] && P.sz[g] <= kBigInverse) fill_blocks(g);
@@ -316,7 +327,8 @@ void build_triplan(const std::vector<const SupernodalFactor*>& F, const std::vec
     }
     P.sn_begin[nh] = (int)P.sn_list.size();
     for (int h = 0; h < nh; ++h)
-        if ((bool)P.fused[h] != (bool)fused_h[h]) throw std::runtime_error("triangular-solve plan: fused heights disagree");
+        if ((bool)P.fused[h] != (bool)fused_h[h] || (bool)P.big[h] != (bool)big_h[h])
+            throw std::runtime_error("triangular-solve plan: fused or big heights disagree");
     // runs of block-sharing supernodes (sn_list is ordered by block offsets within each height)
     P.run_begin.assign(nh + 1, 0);
     P.run_max.assign(nh, 1);
