@@ -384,3 +384,31 @@ def test_topology_downsample_rule(case, box32d, sphere64):
         assert exp_act.sum() > 0 and exp_dir.sum() > 0
         # every coarse cell that receives a fine Active, Dirichlet or Interface cell is on a page
         assert int(act.sum()) == int(exp_act.sum()) and int(hard.sum()) == int(exp_dir.sum())
+
+
+def test_sphere256_single_precision_oracle_fixture():
+    """The 256^3 workload's two recorded oracle runs (tests/golden/make_sphere256.py):
+    double-precision sigma solves (sphere256_sub32.json) and the single-precision
+    rounding of the reference's PARDISO (sphere256_sub32_single.json). They agree to
+    better than 1e-3 relative per iteration over the first 12 iterations and then move
+    apart by tens of percent (the late iterations are chaotic in the rounding of the
+    sigma solves), within 1e-3 of the initial residual throughout; the iteration counts
+    differ by two (62 and 64). So a float-factor solver that lands within +-1 of either
+    count rounds as faithfully as the reference's own solver can be reproduced."""
+    import json
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    paths = [os.path.join(here, f) for f in ("sphere256_sub32.json", "sphere256_sub32_single.json")]
+    if not all(os.path.exists(p) for p in paths):
+        pytest.skip("256^3 oracle fixtures not generated")
+    d, s = (json.load(open(p)) for p in paths)
+    assert d["tolerance"] == s["tolerance"] and d["n_pages"] == s["n_pages"]
+    assert (d["iterations"], s["iterations"]) == (62, 64)
+    hd, hs = np.array(d["history"]), np.array(s["history"])
+    n = min(len(hd), len(hs))
+    rel = np.abs(hs[:n] - hd[:n]) / hd[:n]
+    assert hd[0] == hs[0]
+    assert rel[:12].max() < 1e-3
+    assert rel[12:].max() > 0.1
+    assert (np.abs(hs[:n] - hd[:n]) / hd[0]).max() < 1e-3
+    assert hd[-1] <= d["tolerance"] and hs[-1] <= s["tolerance"]
