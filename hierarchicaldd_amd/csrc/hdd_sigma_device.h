@@ -37,9 +37,10 @@ struct TriPlan {
     DBuf<int> run_q, run_n, orig_of, s0, sz, nr, roff, srows, sn_of, sn_prob, sn_list, sl_sn, sl_r0, rowlist, inc_ptr, ci_sn, ci_a0, ci_a1, bi_sn, bi_c, bi_j0;
     DBuf<long long> loff, doff, cofs, cpos, bofs, woff;
     DBuf<float> diag, diagT, l21, c, bpart, t, y, w21;
-    // fused heights use W = L21 Linv (one phase less per workgroup; float rounding differs
-    // from the L21 form): on unless a fused form is forced (HDD_FUSED_WAVE) or the solves
-    // are refined (the exact mode keeps the L21 form)
+    // fused heights and heights with big supernodes use W = L21 Linv (one phase less per
+    // fused workgroup, one launch less per big height and pass; float rounding differs from
+    // the L21 form): on unless a fused form is forced (HDD_FUSED_WAVE) or the solves are
+    // refined (the exact mode keeps the L21 form)
     bool wform = false;
     bool use_w() const { return wform && refine == 0; }
     // prob_of[m]: problem of factor m (empty: all problem 0)

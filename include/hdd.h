@@ -363,12 +363,14 @@ hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* byte
  * HDD_BOTTOM_LOCAL=0 turns it off), else 0, out[16] tree heights of the
  * triangular-solve plans done by one fused launch per height (a workgroup or
  * a wavefront per run of supernodes), out[17] how many of those use the
- * wavefront form. By default the fused heights use the W form (their
- * sub-diagonal blocks stored as W = L21 Linv, one dependent phase less per
- * workgroup) with a workgroup per run; HDD_FUSED_WAVE=1 / 0 in the environment
- * forces the L21 form with a wavefront / a workgroup per run (the same bits
- * as each other; the W form rounds differently), and refined solves
- * (hdd_set_sigma_refinement) use the L21 form.
+ * wavefront form. By default the fused heights and the heights with big
+ * supernodes use the W form (their sub-diagonal blocks stored as
+ * W = L21 Linv: one dependent phase less per fused workgroup, two launches
+ * instead of three per big height and pass) with a workgroup per run;
+ * HDD_FUSED_WAVE=1 / 0 in the environment forces the L21 form with a
+ * wavefront / a workgroup per run (the same bits as each other; the W form
+ * rounds differently), and refined solves (hdd_set_sigma_refinement) use the
+ * L21 form.
  * n = number of slots provided (up to 18). No reference counterpart. */
 hdd_status hdd_stats(const hdd_ctx* ctx, double* out, int n);
 
