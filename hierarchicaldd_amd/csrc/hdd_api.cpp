@@ -527,6 +527,21 @@ hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* byt
     })
 }
 
+hdd_status hdd_profile_sweep_grid(const hdd_problem_desc* d, int reps, double* avg_ms, double* bytes) {
+    std::string why;
+    if (!d || reps <= 0 || !avg_ms || !desc_ok(d, why)) {
+        g_create_error = "hdd_profile_sweep_grid: " + (d && reps > 0 && avg_ms ? why : std::string("invalid arguments"));
+        return HDD_ERR_ARG;
+    }
+    try {
+        *avg_ms = profile_sweep_pages({d->size[0], d->size[1], d->size[2]}, (int)d->n_pages, d->page_offsets, d->flags, reps, bytes);
+        return HDD_OK;
+    } catch (std::exception& e) {
+        g_create_error = e.what();
+        return HDD_ERR_DEVICE;
+    }
+}
+
 hdd_status hdd_profile_copy(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes) {
     if (ctx && ctx->solver->host_only) { ctx->err = "context was created host-only"; return HDD_ERR_ARG; }
     if (!ctx || reps <= 0 || !avg_ms) return HDD_ERR_ARG;

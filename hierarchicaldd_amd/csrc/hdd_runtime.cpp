@@ -346,9 +346,11 @@ Solver::~Solver() {
     if (mine) (void)hipStreamDestroy(mine);
 }
 
-void Solver::upload_levels() {
+// one level's flags, face-neighbour and origin tables and block lookup table on the device,
+// with its two smoothing buffers, rhs and residual
+static void upload_level(LevelBuffers& L) {
     static const int d[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-    for (auto& L : lv) {
+    {
         const PagedLevel& H = L.host;
         const int np = H.npages();
         std::vector<uint32_t> f(H.flags);
@@ -370,6 +372,49 @@ void Solver::upload_levels() {
         L.b.alloc(L.cells()); L.r.alloc(L.cells());
         L.cur = 0;
     }
+}
+
+void Solver::upload_levels() {
+    for (auto& L : lv) upload_level(L);
+}
+
+double profile_sweep_pages(Int3 size, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, int reps, double* bytes) {
+    LevelBuffers L;
+    L.host.init(size);
+    for (int i = 0; i < n_pages; ++i) {
+        int bx, by, bz;
+        page_coords(page_offsets[i] >> 12, bx, by, bz);
+        const int q = L.host.add_page(bx, by, bz);
+        std::copy(flags + (size_t)i * kPageCells, flags + (size_t)(i + 1) * kPageCells, L.host.flags.begin() + (size_t)q * kPageCells);
+    }
+    L.host.sort_pages();
+    set_face_flags(L.host);
+    upload_level(L);
+    upload_constants();
+    hipStream_t s = nullptr;
+    hdd_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = 0.f;
+    try {
+        hdd_check(hipEventCreate(&a), "hipEventCreate");
+        hdd_check(hipEventCreate(&b), "hipEventCreate");
+        const DevLevel v = L.view();
+        launch_smooth(v, L.u[0].p, L.u[1].p, L.b.p, true, true, s);  // warm
+        hdd_check(hipEventRecord(a, s), "hipEventRecord");
+        for (int i = 0; i < reps; ++i) launch_smooth(v, L.u[i & 1].p, L.u[1 - (i & 1)].p, L.b.p, true, true, s);
+        hdd_check(hipEventRecord(b, s), "hipEventRecord");
+        hdd_check(hipEventSynchronize(b), "hipEventSynchronize");
+        hdd_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    } catch (...) {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    (void)hipEventDestroy(a); (void)hipEventDestroy(b);
+    (void)hipStreamDestroy(s);
+    if (bytes) *bytes = 16.0 * kPageCells * L.host.npages();
+    return ms / reps;
 }
 
 float* Solver::channel(int c) {

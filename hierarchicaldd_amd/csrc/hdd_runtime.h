@@ -104,6 +104,11 @@ struct LevelBuffers {
     size_t cells() const { return (size_t)(host.npages() + 1) * kPageCells; }
 };
 
+// Measurement: mean duration (ms) of one level-0 damped-Jacobi sweep (k_smooth, FP32) of a
+// page set on its own (the given flags plus their face bits, no set-up of the solver), over
+// `reps` back-to-back sweeps timed with HIP events; bytes = 16 B per page slot
+double profile_sweep_pages(Int3 size, int n_pages, const uint64_t* page_offsets, const uint32_t* flags, int reps, double* bytes);
+
 struct PcgResult {
     int iterations = 0;                // negative: not converged within max_it
     bool converged = false;            // the stopping test passed (iterations alone cannot say so at 0)

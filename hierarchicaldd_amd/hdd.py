@@ -60,7 +60,7 @@ EXPORTS = ["hdd_create", "hdd_create_host_only", "hdd_destroy", "hdd_last_error"
            "hdd_inner_product", "hdd_convergence_norm", "hdd_project", "hdd_vector_copy", "hdd_vector_copy2",
            "hdd_vector_plus_equal", "hdd_vector_minus_equal", "hdd_vector_scale", "hdd_apply_preconditioner_channels",
            "hdd_set_pcg_precision", "hdd_get_x_f64", "hdd_set_dot_order", "hdd_inner_product_noalloc", "hdd_batches_pcg_solve",
-           "hdd_set_host_threads"]
+           "hdd_set_host_threads", "hdd_profile_sweep_grid"]
 
 
 def lib():
@@ -422,6 +422,18 @@ class DDBatch(DDSolver):
         out = np.zeros((self.problem_pages(p), 256), dtype=np.float32)
         self._check(lib().hdd_batch_get_x(self.h, p, _ptr(out)))
         return out
+
+
+def profile_sweep_grid(dom, cfg, reps: int = 50):
+    """Mean duration (ms) of one FP32 level-0 sweep of a domain's page set on its own
+    (hdd_profile_sweep_grid: no solver set-up) and its algorithmic bytes."""
+    d, keep = _desc(dom, cfg)
+    ms, by = C.c_double(), C.c_double()
+    st = lib().hdd_profile_sweep_grid(C.byref(d), int(reps), C.byref(ms), C.byref(by))
+    if st != 0:
+        raise HddError(f"hdd_profile_sweep_grid failed ({st}): {lib().hdd_last_error(None).decode()}")
+    del keep
+    return ms.value, by.value
 
 
 def set_host_threads(n: int):

@@ -336,6 +336,14 @@ hdd_status hdd_setup_seconds(const hdd_ctx* ctx, double* seconds);
  * reference counterpart. */
 hdd_status hdd_profile_sweep(hdd_ctx* ctx, int reps, double* avg_ms, double* bytes_per_launch);
 
+/* Measurement without a context: the same FP32 sweep on the level-0 grid of a
+ * page set alone (desc's size, pages and flags, plus their face bits; no
+ * divider, interface operators or factorisation), `reps` back-to-back launches
+ * on a stream of its own bracketed by HIP events: mean duration of one launch
+ * and 16 B per page slot. Lets a grid larger than the Infinity Cache (512^3
+ * sparse: 572 MB) be timed without its set-up. No reference counterpart. */
+hdd_status hdd_profile_sweep_grid(const hdd_problem_desc* desc, int reps, double* avg_ms, double* bytes_per_launch);
+
 /* Measurement: runs `reps` plain copies of one level-0 float channel into the
  * other smoothing buffer (the calibration kernel for hardware byte counters:
  * it reads and writes 4 B per page slot); returns the mean duration and the

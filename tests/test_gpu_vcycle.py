@@ -122,3 +122,22 @@ def test_bottom_page_local_bitwise(name, fp16):
     assert l0 == 0 and l1 == 1
     assert same_bits(z0, z1)
     assert i0 == i1 and np.array_equal(h0, h1) and same_bits(x0, x1)
+
+
+def test_sweep_timer_without_setup():
+    """hdd_profile_sweep_grid times the level-0 sweep of a page set without the solver's
+    set-up: one launch moves 16 B per page slot, and its time is that of the same sweep
+    inside a context to within a factor of two (both are back-to-back launches timed with
+    HIP events; the page set alone lacks only the divider's interface flags)."""
+    from hierarchicaldd_amd import hdd
+    dom, cfg = domain.config0()
+    ms_grid, by_grid = hdd.profile_sweep_grid(dom, cfg, 50)
+    g = hdd.DDSolver(dom, cfg)
+    try:
+        ms_ctx, by_ctx = g.profile_sweep(50)
+    finally:
+        g.close()
+    print(f"[sweep timer] page set alone {ms_grid * 1e3:.1f} us, in a context {ms_ctx * 1e3:.1f} us", flush=True)
+    assert by_grid == by_ctx == 16.0 * 256 * dom.n_pages
+    assert ms_grid > 0 and ms_ctx > 0
+    assert 0.5 < ms_grid / ms_ctx < 2.0
