@@ -363,6 +363,60 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn_tiles_w(int q0, int nq, int 
     bwd_tile_item(it, bi_sn, bi_c, bi_j0, sz, nr, woff, roff, srows, bofs, w21, z, bpart, sn_prob, active);
 }
 
+// Row products of the fused kernels: rows r = threadIdx.x / G + i 256/G below
+// nrows of the row-major block A (row stride n), each row's G lanes summing
+// A[r][k] v[k][j] over k = k0(r) + lane + u G (u ascending, k < kend(r)) for
+// the members j < cnt, then the lane-group tree; out(r, j, sum, leader) takes
+// the sums. Each round issues the loads of kRows rows before its first
+// product, so a lane group waits on memory once per kRows rows; every row's
+// sum keeps its order, so the round size changes no bit. Two rows per round
+// for G >= 8 (one for G = 4, whose rows hold 32 loads per lane) took the
+// 256^3 level-0 Arr solve from 1,142 to 918 us; four rows (fewer waves per
+// SIMD) read 963 us, two rows at G = 4 1,113 us.
+template <int G, int M, class K0, class KEnd, class Out>
+__device__ __forceinline__ void row_products(const float* __restrict__ A, int n, int nrows, int cnt,
+                                             const float (*__restrict__ v)[M], K0 k0, KEnd kend, Out out) {
+    constexpr int kU = TriPlanHost::kSliceRows / G, kStep = 256 / G;
+    constexpr int kRows = G >= 8 ? 2 : 1;
+    const int lane = threadIdx.x & (G - 1);
+    for (int r = (int)threadIdx.x / G; r < nrows; r += kStep * kRows) {
+        float d[kRows][kU];
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) {
+            const int ri = r + i * kStep, b = k0(ri), e = kend(ri);
+            const float* Ar = A + (long long)ri * n;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int k = b + lane + u * G;
+                d[i][u] = (ri < nrows && k < e) ? Ar[k] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kRows; ++i) {
+            const int ri = r + i * kStep, b = k0(ri), e = kend(ri);
+            float acc[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) acc[j] = 0.f;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int k = b + lane + u * G;
+                if (ri < nrows && k < e) {
+                    float vk[M];
+#pragma unroll
+                    for (int j = 0; j < M; ++j) vk[j] = v[k][j];
+#pragma unroll
+                    for (int j = 0; j < M; ++j)
+                        if (j < cnt) acc[j] += d[i][u] * vk[j];
+                }
+            }
+            if (kRows > 1 && ri >= nrows) break;  // uniform over the lane group
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < cnt) out(ri, j, group_sum<G>(acc[j]), lane == 0);
+        }
+    }
+}
+
 // Heights whose supernodes all have <= kSliceRows (128) rows: one workgroup per
 // run of up to M supernodes that share their blocks (members of identical
 // factors; runs of one otherwise) does the whole height's work for them in one
@@ -405,63 +459,21 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
         ts[k][j] = v;
     }
     __syncthreads();
-    const float* D = diag + doff[g0];
-    const int lane = threadIdx.x & (G - 1);
-    constexpr int kU = TriPlanHost::kSliceRows / G;  // fixed trip count: every load of a row is issued at once
-    for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
-        const float* Dr = D + (long long)r * n;
-        float acc[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) acc[j] = 0.f;
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int k = lane + u * G;
-            if (k <= r) {
-                const float d = Dr[k];
-                float tk[M];
-#pragma unroll
-                for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
-#pragma unroll
-                for (int j = 0; j < M; ++j)
-                    if (j < cnt) acc[j] += d * tk[j];
+    // y = Linv t (entries k <= r); a member past the run's count is never read
+    row_products<G, M>(
+        diag + doff[g0], n, n, cnt, ts, [](int) { return 0; }, [](int r) { return r + 1; },
+        [&](int r, int j, float v, bool leader) {
+            if (leader) {
+                if (!WF) ys[r][j] = v;
+                y[base_s[j] + r] = v;
             }
-        }
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < cnt) {
-                const float v = group_sum<G>(acc[j]);
-                if (lane == 0) { ys[r][j] = v; y[base_s[j] + r] = v; }
-            } else if (lane == 0) {
-                ys[r][j] = 0.f;
-            }
-    }
+        });
     if (!WF) __syncthreads();  // WF: the contributions read t, complete since the gather
-    const float* L = l21 + loff[g0];
-    for (int a = (int)threadIdx.x / G; a < m; a += 256 / G) {
-        const float* La = L + (long long)a * n;
-        float acc[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) acc[j] = 0.f;
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int k = lane + u * G;
-            if (k < n) {
-                const float l = La[k];
-                float yk[M];
-#pragma unroll
-                for (int j = 0; j < M; ++j) yk[j] = WF ? ts[k][j] : ys[k][j];
-#pragma unroll
-                for (int j = 0; j < M; ++j)
-                    if (j < cnt) acc[j] += l * yk[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < cnt) {
-                const float v = group_sum<G>(acc[j]);
-                if (lane == 0) c[cofs[sn_list[q0 + j]] + a] = v;
-            }
-    }
+    row_products<G, M>(
+        l21 + loff[g0], n, m, cnt, WF ? ts : ys, [](int) { return 0; }, [n](int) { return n; },
+        [&](int a, int j, float v, bool leader) {
+            if (leader) c[cofs[sn_list[q0 + j]] + a] = v;
+        });
 }
 
 template <int G, int M, bool WF>
@@ -515,35 +527,13 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
     for (int j = 0; j < M; ++j) { acc0[j] = 0.f; acc1[j] = 0.f; }
     __syncthreads();
     const float* DT = diagT + doff[g0];
-    const int gl = threadIdx.x & (G - 1);
-    constexpr int kU = TriPlanHost::kSliceRows / G;
     // WF: Linv^T y into z1, rows and sums as in the diagonal step below
     auto diag_of_y = [&]() {
-        for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
-            const float* Dr = DT + (long long)r * n;
-            float acc[M];
-#pragma unroll
-            for (int j = 0; j < M; ++j) acc[j] = 0.f;
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int k = r + gl + u * G;
-                if (k < n) {
-                    const float d = Dr[k];
-                    float tk[M];
-#pragma unroll
-                    for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
-#pragma unroll
-                    for (int j = 0; j < M; ++j)
-                        if (j < cnt) acc[j] += d * tk[j];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < M; ++j)
-                if (j < cnt) {
-                    const float v = group_sum<G>(acc[j]);
-                    if (gl == 0) z1[r][j] = v;
-                }
-        }
+        row_products<G, M>(
+            DT, n, n, cnt, ts, [](int r) { return r; }, [n](int) { return n; },
+            [&](int r, int j, float v, bool leader) {
+                if (leader) z1[r][j] = v;
+            });
     };
     if (WF) {
         // t = y, visible after the first chunk's barrier (or the one below without sub-diagonal rows)
@@ -603,31 +593,14 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
         ts[k][j] = j < cnt ? y[base_s[j] + k] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k])) : 0.f;
     }
     __syncthreads();
-    for (int r = (int)threadIdx.x / G; r < n; r += 256 / G) {
-        const float* Dr = DT + (long long)r * n;
-        float acc[M];
-#pragma unroll
-        for (int j = 0; j < M; ++j) acc[j] = 0.f;
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            const int k = r + gl + u * G;
-            if (k < n) {
-                const float d = Dr[k];
-                float tk[M];
-#pragma unroll
-                for (int j = 0; j < M; ++j) tk[j] = ts[k][j];
-#pragma unroll
-                for (int j = 0; j < M; ++j)
-                    if (j < cnt) acc[j] += d * tk[j];
+    row_products<G, M>(
+        DT, n, n, cnt, ts, [](int r) { return r; }, [n](int) { return n; },
+        [&](int r, int j, float v, bool leader) {
+            if (leader) {
+                z[base_s[j] + r] = v;
+                x[orig_of[base_s[j] + r]] = v;
             }
-        }
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < cnt) {
-                const float v = group_sum<G>(acc[j]);
-                if (gl == 0) { z[base_s[j] + r] = v; x[orig_of[base_s[j] + r]] = v; }
-            }
-    }
+        });
 }
 
 template <int G, int M, bool WF>
