@@ -370,14 +370,16 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn_tiles_w(int q0, int nq, int 
 // the sums. Each round issues the loads of kRows rows before its first
 // product, so a lane group waits on memory once per kRows rows; every row's
 // sum keeps its order, so the round size changes no bit. Two rows per round
-// for G >= 8 (one for G = 4, whose rows hold 32 loads per lane) took the
-// 256^3 level-0 Arr solve from 1,142 to 918 us; four rows (fewer waves per
-// SIMD) read 963 us, two rows at G = 4 1,113 us.
+// for G >= 8 and one member (one for G = 4, whose rows hold 32 loads per
+// lane) took the 256^3 level-0 Arr solve from 1,142 to 918 us; four rows
+// (fewer waves per SIMD) read 963 us, two rows at G = 4 1,113 us. Runs of
+// four members keep one row per round (their accumulators already fill the
+// registers): level-0 Aii 759 us with two rows, 717 us with one.
 template <int G, int M, class K0, class KEnd, class Out>
 __device__ __forceinline__ void row_products(const float* __restrict__ A, int n, int nrows, int cnt,
                                              const float (*__restrict__ v)[M], K0 k0, KEnd kend, Out out) {
     constexpr int kU = TriPlanHost::kSliceRows / G, kStep = 256 / G;
-    constexpr int kRows = G >= 8 ? 2 : 1;
+    constexpr int kRows = G >= 8 && M == 1 ? 2 : 1;
     const int lane = threadIdx.x & (G - 1);
     for (int r = (int)threadIdx.x / G; r < nrows; r += kStep * kRows) {
         float d[kRows][kU];
