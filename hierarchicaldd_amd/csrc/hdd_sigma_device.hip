@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn_tiles_w(int q0, int nq, int 
 // nrows of the row-major block A (row stride n), each row's G lanes summing
 // A[r][k] v[k][j] over k = k0(r) + lane + u G (u ascending, k < kend(r)) for
 // the members j < cnt, then the lane-group tree; out(r, j, sum, leader) takes
-// the sums. Each round issues the loads of kRows rows before its first
+// the sums (S: the kernel's largest supernode, 64 or 128 rows). Each round
+// issues the loads of kRows rows before its first
 // product, so a lane group waits on memory once per kRows rows; every row's
 // sum keeps its order, so the round size changes no bit. Two rows per round
 // for G >= 8 and one member (one for G = 4, whose rows hold 32 loads per
@@ -375,10 +376,10 @@ __global__ void __launch_bounds__(256) k_tri_bwd_sn_tiles_w(int q0, int nq, int 
 // (fewer waves per SIMD) read 963 us, two rows at G = 4 1,113 us. Runs of
 // four members keep one row per round (their accumulators already fill the
 // registers): level-0 Aii 759 us with two rows, 717 us with one.
-template <int G, int M, class K0, class KEnd, class Out>
+template <int G, int M, int S, class K0, class KEnd, class Out>
 __device__ __forceinline__ void row_products(const float* __restrict__ A, int n, int nrows, int cnt,
                                              const float (*__restrict__ v)[M], K0 k0, KEnd kend, Out out) {
-    constexpr int kU = TriPlanHost::kSliceRows / G, kStep = 256 / G;
+    constexpr int kU = S / G, kStep = 256 / G;
     constexpr int kRows = G >= 8 && M == 1 ? 2 : 1;
     const int lane = threadIdx.x & (G - 1);
     for (int r = (int)threadIdx.x / G; r < nrows; r += kStep * kRows) {
@@ -427,7 +428,7 @@ __device__ __forceinline__ void row_products(const float* __restrict__ A, int n,
 // forward: t = b - (incidence sums) -> y = Linv t (LDS) -> c_g = L21_g y.
 // With WF the block passed as l21 is W_g = L21_g Linv_g (formed in double at
 // set-up), so c_g = W_g t needs only t: both products run in one phase.
-template <int G, int M, bool WF>
+template <int G, int M, bool WF, int S>
 __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -438,8 +439,8 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
                                                        const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     // member-minor vectors: one row's values of all members are one (M = 8: two 16-byte) LDS read
-    __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
-    __shared__ __align__(16) float ys[TriPlanHost::kSliceRows][M];
+    __shared__ __align__(16) float ts[S][M];
+    __shared__ __align__(16) float ys[S][M];
     __shared__ int base_s[M];
     const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
@@ -462,7 +463,7 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
     }
     __syncthreads();
     // y = Linv t (entries k <= r); a member past the run's count is never read
-    row_products<G, M>(
+    row_products<G, M, S>(
         diag + doff[g0], n, n, cnt, ts, [](int) { return 0; }, [](int r) { return r + 1; },
         [&](int r, int j, float v, bool leader) {
             if (leader) {
@@ -471,14 +472,14 @@ __device__ __forceinline__ void fwd_small_run(int run, int r0, const int* __rest
             }
         });
     if (!WF) __syncthreads();  // WF: the contributions read t, complete since the gather
-    row_products<G, M>(
+    row_products<G, M, S>(
         l21 + loff[g0], n, m, cnt, WF ? ts : ys, [](int) { return 0; }, [n](int) { return n; },
         [&](int a, int j, float v, bool leader) {
             if (leader) c[cofs[sn_list[q0 + j]] + a] = v;
         });
 }
 
-template <int G, int M, bool WF>
+template <int G, int M, bool WF, int S>
 __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -489,7 +490,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, int nruns, const 
                                                        const float* __restrict__ l21, float* __restrict__ c, float* __restrict__ y,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     for (int run = blockIdx.x; run < nruns; run += gridDim.x) {
-        fwd_small_run<G, M, WF>(run, r0, run_q, run_n, sn_list, s0, sz, nr, doff, diag, inc_ptr, cpos, orig_of, b, loff, cofs, l21, c, y, sn_prob, active);
+        fwd_small_run<G, M, WF, S>(run, r0, run_q, run_n, sn_list, s0, sz, nr, doff, diag, inc_ptr, cpos, orig_of, b, loff, cofs, l21, c, y, sn_prob, active);
         __syncthreads();  // the next run reuses the LDS vectors
     }
 }
@@ -499,7 +500,7 @@ __global__ void __launch_bounds__(256) k_tri_fwd_small(int r0, int nruns, const 
 // columns j and j + 64; LDS reduction) -> z = Linv^T t. With WF the block is
 // W_g = L21_g Linv_g and z = Linv^T y - W^T z[rows]: the diagonal product needs
 // no ancestor value and runs beside the tile sums.
-template <int G, int M, bool WF>
+template <int G, int M, bool WF, int S>
 __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -511,9 +512,9 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     constexpr int kChunk = 128;  // a multiple of 4: the rows' order per wavefront does not depend on it
     __shared__ __align__(16) float zr[kChunk][M];  // member-minor, as ts
-    __shared__ float part[4][M][TriPlanHost::kSliceRows];
-    __shared__ __align__(16) float ts[TriPlanHost::kSliceRows][M];
-    __shared__ __align__(16) float z1[WF ? TriPlanHost::kSliceRows : 1][M];  // WF: Linv^T y
+    __shared__ float part[4][M][S];
+    __shared__ __align__(16) float ts[S][M];
+    __shared__ __align__(16) float z1[WF ? S : 1][M];  // WF: Linv^T y
     __shared__ int base_s[M], roff_s[M];
     const int q0 = run_q[r0 + run], cnt = run_n[r0 + run];
     const int g0 = sn_list[q0], n = sz[g0], m = nr[g0];
@@ -531,7 +532,7 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
     const float* DT = diagT + doff[g0];
     // WF: Linv^T y into z1, rows and sums as in the diagonal step below
     auto diag_of_y = [&]() {
-        row_products<G, M>(
+        row_products<G, M, S>(
             DT, n, n, cnt, ts, [](int r) { return r; }, [n](int) { return n; },
             [&](int r, int j, float v, bool leader) {
                 if (leader) z1[r][j] = v;
@@ -595,7 +596,7 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
         ts[k][j] = j < cnt ? y[base_s[j] + k] - ((part[0][j][k] + part[1][j][k]) + (part[2][j][k] + part[3][j][k])) : 0.f;
     }
     __syncthreads();
-    row_products<G, M>(
+    row_products<G, M, S>(
         DT, n, n, cnt, ts, [](int r) { return r; }, [n](int) { return n; },
         [&](int r, int j, float v, bool leader) {
             if (leader) {
@@ -605,7 +606,7 @@ __device__ __forceinline__ void bwd_small_run(int run, int r0, const int* __rest
         });
 }
 
-template <int G, int M, bool WF>
+template <int G, int M, bool WF, int S>
 __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, int nruns, const int* __restrict__ run_q, const int* __restrict__ run_n,
                                                        const int* __restrict__ sn_list, const int* __restrict__ s0,
                                                        const int* __restrict__ sz, const int* __restrict__ nr,
@@ -616,7 +617,7 @@ __global__ void __launch_bounds__(256) k_tri_bwd_small(int r0, int nruns, const 
                                                        float* __restrict__ z, float* __restrict__ x,
                                                        const int* __restrict__ sn_prob, const int* __restrict__ active) {
     for (int run = blockIdx.x; run < nruns; run += gridDim.x) {
-        bwd_small_run<G, M, WF>(run, r0, run_q, run_n, sn_list, s0, sz, nr, roff, srows, loff, l21, doff, diagT, orig_of, y, z, x, sn_prob, active);
+        bwd_small_run<G, M, WF, S>(run, r0, run_q, run_n, sn_list, s0, sz, nr, roff, srows, loff, l21, doff, diagT, orig_of, y, z, x, sn_prob, active);
         __syncthreads();  // the next run reuses the LDS vectors
     }
 }
@@ -1201,25 +1202,33 @@ static constexpr int kRunM = TriPlanHost::kRunMembers;
         }                                                                                                       \
     } while (0)
 
-#define HDD_RUN_WF_DISPATCH(G, M, WF, KERNEL, BLOCKS, STREAM, ...)                                                  \
+#define HDD_RUN_WF_DISPATCH(G, M, WF, S64, KERNEL, BLOCKS, STREAM, ...)                                             \
     do {                                                                                                            \
         switch (G) {                                                                                                \
-            case 4: hipLaunchKernelGGL((KERNEL<4, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break;   \
-            case 8: hipLaunchKernelGGL((KERNEL<8, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break;   \
-            case 16: hipLaunchKernelGGL((KERNEL<16, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
-            case 32: hipLaunchKernelGGL((KERNEL<32, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
-            default: hipLaunchKernelGGL((KERNEL<64, M, WF>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            case 4:                                                                                                 \
+                if (S64) hipLaunchKernelGGL((KERNEL<4, M, WF, 64>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<4, M, WF, 128>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__);    \
+                break;                                                                                              \
+            case 8:                                                                                                 \
+                if (S64) hipLaunchKernelGGL((KERNEL<8, M, WF, 64>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); \
+                else hipLaunchKernelGGL((KERNEL<8, M, WF, 128>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__);    \
+                break;                                                                                              \
+            case 16: hipLaunchKernelGGL((KERNEL<16, M, WF, 128>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            case 32: hipLaunchKernelGGL((KERNEL<32, M, WF, 128>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<64, M, WF, 128>), dim3(BLOCKS), dim3(256), 0, STREAM, __VA_ARGS__); break; \
         }                                                                                                           \
     } while (0)
-// the fused workgroup kernels: run length (runs of one or of up to kRunM) x block form (L21 or W)
+// the fused workgroup kernels: run length (runs of one or of up to kRunM) x block form (L21 or W);
+// at 4 and 8 lanes per row, heights whose supernodes have at most 64 rows take the 64-row
+// form (half the fixed trip count and registers, the same sums: level-0 Aii 711 -> 691 us)
 #define HDD_FUSED_DISPATCH(H, WF, KERNEL, BLOCKS, STREAM, ...)                                                       \
     do {                                                                                                            \
         if (run_max[H] > 1) {                                                                                       \
-            if (WF) HDD_RUN_WF_DISPATCH(G[H], kRunM, true, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                    \
-            else HDD_RUN_WF_DISPATCH(G[H], kRunM, false, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                      \
+            if (WF) HDD_RUN_WF_DISPATCH(G[H], kRunM, true, max_sz[H] <= 64, KERNEL, BLOCKS, STREAM, __VA_ARGS__);    \
+            else HDD_RUN_WF_DISPATCH(G[H], kRunM, false, max_sz[H] <= 64, KERNEL, BLOCKS, STREAM, __VA_ARGS__);      \
         } else {                                                                                                    \
-            if (WF) HDD_RUN_WF_DISPATCH(G[H], 1, true, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                        \
-            else HDD_RUN_WF_DISPATCH(G[H], 1, false, KERNEL, BLOCKS, STREAM, __VA_ARGS__);                          \
+            if (WF) HDD_RUN_WF_DISPATCH(G[H], 1, true, max_sz[H] <= 64, KERNEL, BLOCKS, STREAM, __VA_ARGS__);        \
+            else HDD_RUN_WF_DISPATCH(G[H], 1, false, max_sz[H] <= 64, KERNEL, BLOCKS, STREAM, __VA_ARGS__);          \
         }                                                                                                           \
     } while (0)
 
